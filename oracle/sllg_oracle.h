@@ -1,0 +1,186 @@
+/*
+ * sllg_oracle.h -- CPU FP64 restatement of owlas/magpy's stochastic LLG path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path (magpy_amd/).  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Functions that
+ * belong to this build (the Philox Wiener stream) cite their published
+ * algorithm instead.
+ */
+#ifndef SLLG_ORACLE_H
+#define SLLG_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/constants.hpp:10-12 */
+extern const double ORC_KB, ORC_MU0, ORC_GYROMAG;
+
+/* ---------------- random numbers ---------------- */
+
+/* Philox4x32-10 (Salmon et al., SC'11, "Parallel random numbers: as easy as
+ * 1, 2, 3").  Counter-based: out = bijection_key(ctr). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Map a 32-bit word to the open interval (0,1): (x + 1/2) * 2^-32. */
+double orc_u01(uint32_t x);
+
+/* Wiener stream of one particle.  Normal number j of the stream of particle
+ * `particle` of a member whose seed is `seed` is component (j & 3) of the
+ * Box-Muller transform of Philox4x32-10 with
+ *   key = (seed & 0xffffffff, seed >> 32)
+ *   ctr = (c & 0xffffffff, c >> 32, particle, tag),  c = j >> 2.
+ * Box-Muller pairs (w0,w1) -> (r cos 2 pi u(w1), r sin 2 pi u(w1)),
+ * r = sqrt(-2 ln u(w0)), and likewise (w2,w3).  Writes n normals starting
+ * at index j0. */
+void orc_wiener_normals(uint64_t seed, uint32_t particle, uint32_t tag,
+                        uint64_t j0, size_t n, double *out);
+
+/* std::mt19937_64 + std::normal_distribution<double> (libstdc++), the
+ * generator of RngMtNorm (lib/rng.cpp:14-24, include/rng.hpp:36-37). */
+typedef struct {
+    uint64_t mt[312];
+    int mti;
+    int has_saved;
+    double saved;
+    double mean, stddev;
+} orc_mtnorm;
+void orc_mtnorm_init(orc_mtnorm *g, uint64_t seed, double stddev);
+uint64_t orc_mt64_next(orc_mtnorm *g);
+double orc_mtnorm_get(orc_mtnorm *g);
+
+/* ---------------- LLG terms (lib/llg.cpp) ---------------- */
+void orc_llg_drift(double *deriv, const double *m, double alpha, const double *h);
+void orc_llg_diffusion(double *deriv, const double *m, double sr, double alpha);
+void orc_llg_drift_jacobian(double *jac, const double *m, double a,
+                            const double *h, const double *hj);
+void orc_llg_diffusion_jacobian(double *jac, const double *m, double sr, double alpha);
+void orc_llg_multi_drift(double *deriv, const double *m, const double *alphas,
+                         const double *h, size_t n);
+void orc_llg_multi_diffusion(double *deriv, const double *m, const double *srs,
+                             const double *alphas, size_t n);
+void orc_llg_multi_drift_quasijacobian(double *jac, const double *m, const double *alphas,
+                                       const double *h, const double *hj, size_t n);
+void orc_llg_multi_diffusion_jacobian(double *jac, const double *m, const double *srs,
+                                      const double *alphas, size_t n);
+
+/* ---------------- fields (lib/field.cpp) ---------------- */
+double orc_field_sinusoidal(double t, double h, double f);
+double orc_field_square(double t, double h, double f);
+void orc_field_uniaxial(double *h, const double *m, const double *axis);
+void orc_field_uniaxial_jacobian(double *jac, const double *axis);
+void orc_field_multi_add_uniaxial(double *h, const double *m, const double *axes,
+                                  const double *k_red, size_t n);
+void orc_field_multi_add_uniaxial_jacobian(double *jac, const double *axes,
+                                           const double *k_red, size_t n);
+void orc_field_multi_add_applied_z(double *h, double happ, size_t n);
+double orc_dipolar_prefactor(double ms, double k_av);
+void orc_dipolar_add_p2p_term(double *out, double vj, double rij3, const double *mj,
+                              const double *dist, double prefactor);
+void orc_field_multi_add_dipolar(double *h, double ms, double k_av, const double *v_red,
+                                 const double *m, const double *dists,
+                                 const double *dist_cubes, size_t n);
+
+/* ---------------- distances (lib/distances.cpp) ---------------- */
+void orc_pair_distance_vectors(double *out, const double *points, size_t n);
+void orc_pair_distance_magnitudes(double *out, const double *vecs, size_t n);
+void orc_pair_unit_vectors(double *out, const double *vecs, const double *mags, size_t n);
+
+/* ---------------- integrators ---------------- */
+/* sde(ctx, drift[n], diffusion[n*w], state[n], t) */
+typedef void (*orc_sde_fn)(void *ctx, double *drift, double *diffusion,
+                           const double *state, double t);
+/* sde_jac(ctx, a[n], b[n*w], a'[n*n], b'[n*w*n], state[n], t_a, t_b) */
+typedef void (*orc_sdejac_fn)(void *ctx, double *a, double *b, double *adash,
+                              double *bdash, const double *state, double ta, double tb);
+/* f(ctx, fout[n], jac[n*n], x[n]) */
+typedef void (*orc_fj_fn)(void *ctx, double *fout, double *jac, const double *x);
+
+void orc_heun(double *next, double *drift, double *trial_drift, double *diff,
+              double *trial_diff, const double *cur, const double *w,
+              orc_sde_fn sde, void *ctx, size_t n, size_t wdim, double t, double dt);
+int orc_dgesv(int n, double *a, int *ipiv, double *b);
+int orc_newton_noinv(double *x_root, double *x_tmp, double *jac, int *ipiv, int *lapack_err,
+                     orc_fj_fn fj, void *ctx, const double *x0, int dim, double eps,
+                     size_t max_iter);
+int orc_implicit_midpoint(double *x, double *dwm, double *a_work, double *b_work,
+                          double *adash_work, double *bdash_work, double *x_guess,
+                          double *x_opt_tmp, double *x_opt_jac, int *ipiv,
+                          const double *x0, const double *dw, orc_sdejac_fn sde, void *ctx,
+                          size_t n, size_t wdim, double t, double dt, double eps,
+                          size_t max_iter, int *newton_iters);
+
+double orc_trapezoidal(const double *x, const double *y, size_t n);
+
+/* ---------------- full dynamics (lib/simulation.cpp) ---------------- */
+enum { ORC_FIELD_SINE = 0, ORC_FIELD_SQUARE = 1, ORC_FIELD_CONSTANT = 2 };
+enum { ORC_RNG_MT = 0, ORC_RNG_ARRAY = 1, ORC_RNG_PHILOX = 2 };
+
+typedef struct {
+    int kind;
+    /* ORC_RNG_MT */
+    orc_mtnorm mt;
+    /* ORC_RNG_ARRAY: RngArray (lib/rng.cpp:67-94) */
+    const double *arr;
+    size_t arr_len, arr_stride, arr_i;
+    int exhausted;
+    /* ORC_RNG_PHILOX: 3N draws per step, particle-major */
+    uint64_t seed;
+    uint32_t tag, n_particles;
+    uint64_t step;
+    uint32_t idx;
+} orc_rng;
+
+void orc_rng_init_mt(orc_rng *r, uint64_t seed, double stddev);
+void orc_rng_init_array(orc_rng *r, const double *arr, size_t len, size_t stride);
+void orc_rng_init_philox(orc_rng *r, uint64_t seed, uint32_t tag, uint32_t n_particles);
+double orc_rng_get(orc_rng *r);
+
+/* Per-run diagnostics the reference discards (lib/simulation.cpp:368-369). */
+typedef struct {
+    int64_t n_steps;
+    int64_t newton_iters;      /* total over the run */
+    int64_t newton_failures;   /* steps whose Newton returned non-zero */
+    int rng_exhausted;
+} orc_stats;
+
+/* Reduced-unit overload, lib/simulation.cpp:132-429.  Arrays:
+ * sr, k_red, v_red [n]; axes, m0 [3n]; unit_dists [n*n*3]; red_dists [n*n];
+ * outputs time, field [S]; mx, my, mz [n*S] (particle-major).  Returns 0 on
+ * success, -1 on bad arguments. */
+int orc_full_dynamics_reduced(
+    const double *sr, const double *k_red, const double *v_red, const double *axes,
+    const double *m0, const double *unit_dists, const double *red_dists, size_t n,
+    int field_shape, double happ, double fapp, double average_anisotropy,
+    double average_volume, double damping, double ms, double dt, double end_time,
+    orc_rng *rng, int renorm, int interactions, int use_implicit, double eps,
+    int max_samples, double *time, double *field, double *mx, double *my, double *mz,
+    orc_stats *stats);
+
+/* Physical overload, lib/simulation.cpp:476-624 (seeded RNG replaced by the
+ * caller's choice: rng_kind ORC_RNG_MT uses `seed` as the reference does,
+ * ORC_RNG_PHILOX uses the build's Philox stream with the same seed,
+ * ORC_RNG_ARRAY replays `noise`). */
+int orc_full_dynamics(
+    const double *radius, const double *anisotropy, const double *axes, const double *m0,
+    const double *location, size_t n, double magnetisation, double damping,
+    double temperature, int renorm, int interactions, int use_implicit, double eps,
+    double time_step, double end_time, size_t max_samples, int64_t seed,
+    int field_shape, double field_amplitude, double field_frequency,
+    int rng_kind, const double *noise, size_t noise_len,
+    double *time, double *field, double *mx, double *my, double *mz, orc_stats *stats);
+
+/* Sample schedule of the reduced loop (lib/simulation.cpp:342-406): k[s] is
+ * the step index whose state is recorded as sample s; returns the number of
+ * integration steps the loop takes (k[S-1] + 1). */
+int64_t orc_sample_schedule(double dt, double end_time, size_t max_samples, int64_t *k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
