@@ -1,0 +1,177 @@
+/*
+ * magpy_amd.h -- C ABI of the MI355X sLLG integrator (libmagpy_amd.so).
+ *
+ * Plain pointers and sizes only: no C++ types, no exceptions, no torch types.
+ * Every entry returns MP_OK (0) or an MP_ERR_* code; mp_last_error() gives the
+ * message of the calling thread's last failure.  Inputs are read-only and
+ * caller-owned; outputs are caller-allocated (NULL = not wanted).
+ *
+ * Each entry cites the owlas/magpy interface it replaces.
+ */
+#ifndef MAGPY_AMD_H
+#define MAGPY_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MP_ABI_VERSION 1
+
+/* status codes */
+#define MP_OK 0
+#define MP_ERR_ARG 1          /* invalid argument (message says which)        */
+#define MP_ERR_HIP 2          /* HIP runtime failure                          */
+#define MP_ERR_NODEV 3        /* no HIP device visible                        */
+#define MP_ERR_UNSUPPORTED 4  /* valid request this build does not run        */
+#define MP_ERR_ALLOC 5        /* device or host allocation failed             */
+
+/* applied-field shapes: the values of field::options (include/field.hpp:95-97) */
+#define MP_FIELD_SINE 0
+#define MP_FIELD_SQUARE 1
+#define MP_FIELD_CONSTANT 2
+
+#define MP_SCHEME_HEUN 0              /* integrator::heun, lib/integrators.cpp:372-405 */
+#define MP_SCHEME_IMPLICIT_MIDPOINT 1 /* integrator::implicit_midpoint, :576-651 */
+
+#define MP_PRECISION_FP32 0
+#define MP_PRECISION_FP64 1
+
+/* Wiener increments.  PHILOX: normal j of particle p (index within its
+ * member) is component j%4 of Box-Muller over Philox4x32-10 with
+ * key = (seed lo32, seed hi32), ctr = (j/4 lo32, j/4 hi32, p, noise_tag);
+ * step n uses normals 3n, 3n+1, 3n+2 (x, y, z).  ARRAY: the caller's draws,
+ * the RngArray replacement (lib/rng.cpp:67-94), laid out [step][particle][3]
+ * over the whole ensemble. */
+#define MP_NOISE_PHILOX 0
+#define MP_NOISE_ARRAY 1
+
+/* per-particle status bits */
+#define MP_STATUS_NONFINITE 1u      /* state became NaN/Inf                     */
+#define MP_STATUS_NEWTON_FAIL 2u    /* an implicit step's Newton did not converge */
+
+typedef struct mp_ensemble_spec {
+    int64_t n_members;              /* clusters in the ensemble                  */
+    const int64_t *member_offsets;  /* [n_members+1] CSR offsets into particles  */
+    /* per particle (SI units, as simulation::full_dynamics takes them) */
+    const double *radius;                  /* [P] m                              */
+    const double *anisotropy;              /* [P] J/m^3                          */
+    const double *anisotropy_axis;         /* [P*3] unit vectors                 */
+    const double *magnetisation_direction; /* [P*3] initial unit vectors         */
+    const double *location;                /* [P*3] m                            */
+    /* per member */
+    const double *magnetisation;    /* [M] A/m   */
+    const double *damping;          /* [M]       */
+    const double *temperature;      /* [M] K     */
+    const int32_t *field_shape;     /* [M] MP_FIELD_* */
+    const double *field_amplitude;  /* [M] A/m   */
+    const double *field_frequency;  /* [M] Hz    */
+    const int64_t *seed;            /* [M]       */
+    /* shared by all members */
+    double time_step;               /* s */
+    double end_time;                /* s */
+    int64_t max_samples;            /* >= 1; -1 (never terminates in the reference) is rejected */
+    int32_t renorm;
+    int32_t interactions;
+    int32_t scheme;                 /* MP_SCHEME_* */
+    int32_t precision;              /* MP_PRECISION_* (implicit midpoint runs in FP64) */
+    double implicit_tol;            /* Newton relative tolerance (reference default 1e-9) */
+    int32_t implicit_max_iter;      /* 1000 in the reference (lib/simulation.cpp:204) */
+    int32_t noise;                  /* MP_NOISE_* */
+    const double *noise_array;      /* [noise_steps][P][3] when noise == MP_NOISE_ARRAY */
+    int64_t noise_steps;
+    uint32_t noise_tag;
+    int32_t device;                 /* HIP device ordinal */
+} mp_ensemble_spec;
+
+typedef struct mp_ensemble_out {
+    /* sample times (s) and applied field (A/m) of member 0: [S] */
+    double *time;
+    double *field;
+    /* the same for every member: [M][S] (optional) */
+    double *member_time;
+    double *member_field;
+    /* per-particle magnetisation Ms*m (A/m), particle-major [P][S] as
+     * simulation::results (include/simulation.hpp:32-48) */
+    double *mx, *my, *mz;
+    /* ensemble sum over members and particles of Ms*m per sample: [S][3] */
+    double *ens_m;
+    /* reduced state after the last step: [P][3] */
+    double *final_state;
+    /* per-particle MP_STATUS_* bits: [P] */
+    uint32_t *status;
+    /* filled on return */
+    int64_t n_steps;            /* integration steps taken per member        */
+    int64_t newton_iters;       /* total Newton iterations (implicit)        */
+    int64_t newton_failures;    /* steps whose Newton hit max_iter or a singular matrix */
+    double kernel_ms;           /* device time spent in the step kernels     */
+} mp_ensemble_out;
+
+/* Batched entry point: for every member, the semantics of
+ * simulation::full_dynamics (physical overload, include/simulation.hpp:74-94,
+ * lib/simulation.cpp:476-624) with that member's arrays and seed; replaces the
+ * joblib fan-out of EnsembleModel.simulate (magpy/model.py:202-207). */
+int mp_ensemble_run(const mp_ensemble_spec *spec, mp_ensemble_out *out);
+
+/* One cluster, argument for argument simulation::full_dynamics (physical
+ * overload, include/simulation.hpp:74-94) as core.simulate binds it
+ * (magpy/core.pyx:72-93); outputs particle-major [n][max_samples] like the
+ * vector<results> it returns.  Wiener increments from the Philox stream keyed
+ * by `seed` (MP_NOISE_PHILOX, noise_tag 0). */
+int mp_full_dynamics(const double *radius, const double *anisotropy,
+                     const double *anisotropy_axis, const double *magnetisation_direction,
+                     const double *location, size_t n_particles, double magnetisation,
+                     double damping, double temperature, int renorm, int interactions,
+                     int use_implicit, double eps, double time_step, double end_time,
+                     size_t max_samples, long seed, int field_shape, double field_amplitude,
+                     double field_frequency, int precision, double *time, double *field,
+                     double *mx, double *my, double *mz);
+
+/* Resident run: the ensemble's state and parameters stay in device memory
+ * between calls, so a caller can advance it in blocks of steps (timing,
+ * progress, checkpoints) and collect the same outputs mp_ensemble_run gives.
+ * No counterpart in the reference, whose full_dynamics runs a member to the
+ * end in one call (lib/simulation.cpp:342-406).  A session is used by one
+ * thread at a time. */
+typedef struct mp_session mp_session;
+/* want_* select which outputs are recorded on the device */
+int mp_session_create(const mp_ensemble_spec *spec, int want_trajectories, int want_ensemble,
+                      mp_session **session);
+/* advance every member by up to n_steps (stops at the end of the run);
+ * *steps_done (optional) = steps taken so far after the call */
+int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done);
+/* total steps of the run, and kernel launches / device milliseconds so far */
+int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
+                    double *kernel_ms);
+/* reduced state [P][3] and the step index it belongs to */
+int mp_session_state(mp_session *session, double *state, int64_t *step);
+/* run to the end and fill `out` as mp_ensemble_run would */
+int mp_session_finish(mp_session *session, mp_ensemble_out *out);
+void mp_session_destroy(mp_session *session);
+
+/* Sample schedule of the reduced step loop (lib/simulation.cpp:342-406):
+ * k[s] = step index recorded as sample s; *n_steps = steps the loop takes. */
+int mp_sample_schedule(double dt_reduced, double end_reduced, int64_t max_samples,
+                       int64_t *k, int64_t *n_steps);
+
+/* Fill out[n] with normals j0..j0+n-1 of one particle's Wiener stream
+ * (MP_NOISE_PHILOX definition above) computed on the device in the given
+ * precision.  Exposes the in-kernel generator for parity checks; no
+ * counterpart in the reference (RngMtNorm::get, lib/rng.cpp:24, is what the
+ * stream replaces). */
+int mp_wiener_normals(uint64_t seed, uint32_t particle, uint32_t tag, uint64_t j0, size_t n,
+                      int precision, double *out, int device);
+
+/* get_KB / get_mu0 / get_gamma (magpy/core.pyx:29-34) */
+void mp_constants(double *kb, double *mu0, double *gamma);
+
+const char *mp_last_error(void);
+const char *mp_version(void);
+/* device name, CU count and clock of `device` into buf */
+int mp_device_info(int device, char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

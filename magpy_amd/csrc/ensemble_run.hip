@@ -1,0 +1,900 @@
+// ensemble_run.hip -- host side of the MI355X sLLG integrator and its C ABI.
+//
+// simulation::ensemble_run (simulation.hpp) validates the request, reduces
+// every member to dimensionless units exactly as simulation::full_dynamics
+// does (lib/simulation.cpp:501-583), derives the reference's sample schedule,
+// uploads SoA parameter arrays, and drives the step kernels of
+// sllg_kernels.hpp in blocks of steps on one HIP stream.  A session keeps that
+// state resident between calls.  The extern "C" functions at the end are the
+// boundary declared in include/magpy_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "simulation.hpp"
+#include "sllg_kernels.hpp"
+
+namespace {
+
+// include/constants.hpp:10-12
+constexpr double kKB = 1.38064852e-23;
+constexpr double kMU0 = 1.25663706e-6;
+constexpr double kGYROMAG = 1.76086e11;
+constexpr double kPiHost = 3.14159265358979323846;
+
+thread_local std::string g_last_error;
+
+#define MP_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw simulation::Error(MP_ERR_HIP, std::string(#call) + ": " +                 \
+                                                    hipGetErrorString(e_));                 \
+    } while (0)
+
+void require(bool ok, const std::string &msg)
+{
+    if (!ok) throw simulation::Error(MP_ERR_ARG, msg);
+}
+
+// Device allocation owned by its holder.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    void alloc(size_t count)
+    {
+        release();
+        n = count;
+        if (count) {
+            hipError_t e = hipMalloc(&p, count * sizeof(T));
+            if (e != hipSuccess) {
+                p = nullptr;
+                n = 0;
+                throw simulation::Error(MP_ERR_ALLOC, "hipMalloc of " +
+                                                          std::to_string(count * sizeof(T)) +
+                                                          " bytes failed");
+            }
+        }
+    }
+    void upload(const std::vector<T> &h)
+    {
+        alloc(h.size());
+        if (!h.empty()) MP_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+};
+
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    StreamGuard()
+    {
+        MP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        MP_HIP(hipEventCreate(&e0));
+        MP_HIP(hipEventCreate(&e1));
+    }
+    ~StreamGuard()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+// The applied field of lib/simulation.cpp:552-573 at reduced time t.
+double applied_host(int shape, double happ, double fapp, double t)
+{
+    switch (shape) {
+    case MP_FIELD_SINE: return happ * std::sin(2 * kPiHost * fapp * t);
+    case MP_FIELD_SQUARE: return happ * ((int)(t * fapp * 2) % 2 ? -1 : 1);
+    default: return happ;
+    }
+}
+
+// Steps per kernel launch: long enough to amortise the launch, short enough
+// to keep each launch well under a second.
+int64_t steps_per_launch(int scheme, int precision)
+{
+    if (const char *env = std::getenv("MAGPY_AMD_STEPS_PER_LAUNCH")) {
+        long long v = std::atoll(env);
+        if (v > 0) return (int64_t)v;
+    }
+    if (scheme == MP_SCHEME_IMPLICIT_MIDPOINT) return 1 << 12;
+    return precision == MP_PRECISION_FP32 ? (1 << 17) : (1 << 16);
+}
+
+template <typename T>
+std::vector<T> cast_vec(const std::vector<double> &v)
+{
+    return std::vector<T>(v.begin(), v.end());
+}
+
+template <typename T, int kShape, bool kArray>
+void launch_step(const mp::IndepArgs<T> &a, int scheme, double eps, int max_iter,
+                 unsigned long long *it, unsigned long long *fl, int grid, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 8) {
+        if (scheme == MP_SCHEME_IMPLICIT_MIDPOINT) {
+            mp::ImplicitArgs ia{a, eps, max_iter, it, fl};
+            hipLaunchKernelGGL((mp::implicit_indep_kernel<kShape, kArray>), dim3(grid),
+                               dim3(mp::kBlock), 0, s, ia);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((mp::heun_indep_kernel<T, kShape, kArray>), dim3(grid), dim3(mp::kBlock),
+                       0, s, a);
+}
+
+template <typename T, bool kArray>
+void launch_shape(int shape_kind, const mp::IndepArgs<T> &a, int scheme, double eps,
+                  int max_iter, unsigned long long *it, unsigned long long *fl, int grid,
+                  hipStream_t s)
+{
+    switch (shape_kind) {
+    case mp::kSine: launch_step<T, mp::kSine, kArray>(a, scheme, eps, max_iter, it, fl, grid, s); break;
+    case mp::kSquare: launch_step<T, mp::kSquare, kArray>(a, scheme, eps, max_iter, it, fl, grid, s); break;
+    case mp::kConstant: launch_step<T, mp::kConstant, kArray>(a, scheme, eps, max_iter, it, fl, grid, s); break;
+    default: launch_step<T, mp::kMixed, kArray>(a, scheme, eps, max_iter, it, fl, grid, s); break;
+    }
+}
+
+struct Plan {
+    // particles ordered by schedule group; perm[i] = original particle index
+    std::vector<int64_t> perm;
+    std::vector<int64_t> group_begin;        // [G+1] ranges in perm order
+    std::vector<std::vector<int64_t>> sched; // per group
+    std::vector<int64_t> group_steps;        // per group
+    std::vector<int64_t> member_of;          // per original particle
+    std::vector<simulation::ReducedMember> red;
+};
+
+// Host copies of what the outputs need after the caller's spec is gone.
+struct HostCopy {
+    int64_t M = 0, P = 0, S = 0;
+    std::vector<double> m0, ms;
+    std::vector<int32_t> shape;
+};
+
+// Results of one group copied back to the host.
+struct GroupOut {
+    std::vector<double> state;   // [n][3] in group order
+    std::vector<uint32_t> status;
+    std::vector<double> samples; // [S][3][n] (empty when not recorded)
+    std::vector<double> ens;     // [S][3]   (empty when not recorded)
+    int64_t newton_iters = 0, newton_fail = 0;
+};
+
+struct GroupBase {
+    int64_t n_done = 0, n_total = 0, launches = 0;
+    virtual ~GroupBase() = default;
+    virtual void enqueue(int64_t upto, hipStream_t s) = 0;
+    virtual void state(std::vector<double> &out) = 0;  // [n][3] group order
+    virtual GroupOut collect(hipStream_t s) = 0;
+};
+
+template <typename T>
+struct GroupRun : GroupBase {
+    int n = 0, grid = 0, n_waves = 0, scheme = 0, shape_kind = -1, max_iter = 1000;
+    double eps = 1e-9;
+    bool array_noise = false, want_traj = false, want_ens = false;
+    int64_t S = 0, s_next = 0, spl = 1;
+    std::vector<int64_t> sched;
+    DevBuf<T> d_ex, d_ey, d_ez, d_k, d_al, d_dt, d_sig, d_sr, d_happ, d_mx, d_my, d_mz, d_samples;
+    DevBuf<double> d_fred, d_fdt, d_dtd, d_msw, d_noise, d_partial, d_ens;
+    DevBuf<int> d_shape;
+    DevBuf<uint32_t> d_key0, d_key1, d_pid, d_status;
+    DevBuf<int64_t> d_sched;
+    DevBuf<unsigned long long> d_it, d_fl;
+    mp::IndepArgs<T> a{};
+
+    GroupRun(const simulation::EnsembleSpec &sp, const Plan &plan, int gi, bool traj, bool ens)
+    {
+        const int64_t b = plan.group_begin[(size_t)gi], e = plan.group_begin[(size_t)gi + 1];
+        const int64_t P = sp.member_offsets[sp.n_members];
+        n = (int)(e - b);
+        sched = plan.sched[(size_t)gi];
+        S = (int64_t)sched.size();
+        n_total = plan.group_steps[(size_t)gi];
+        scheme = sp.scheme;
+        eps = sp.implicit_tol;
+        max_iter = sp.implicit_max_iter > 0 ? sp.implicit_max_iter : 1000;
+        array_noise = sp.noise == MP_NOISE_ARRAY;
+        want_traj = traj;
+        want_ens = ens;
+        spl = steps_per_launch(sp.scheme, sp.precision);
+
+        std::vector<double> ex(n), ey(n), ez(n), k(n), al(n), dt(n), sig(n), sr(n), happ(n),
+            fred(n), fdt(n), msw(n), mx(n), my(n), mz(n);
+        std::vector<int> shape(n);
+        std::vector<uint32_t> key0(n), key1(n), pid(n);
+        for (int i = 0; i < n; ++i) {
+            const int64_t p = plan.perm[(size_t)(b + i)];
+            const int64_t m = plan.member_of[(size_t)p];
+            const simulation::ReducedMember &r = plan.red[(size_t)m];
+            const int64_t lp = p - sp.member_offsets[m];
+            ex[i] = sp.anisotropy_axis[3 * p];
+            ey[i] = sp.anisotropy_axis[3 * p + 1];
+            ez[i] = sp.anisotropy_axis[3 * p + 2];
+            mx[i] = sp.magnetisation_direction[3 * p];
+            my[i] = sp.magnetisation_direction[3 * p + 1];
+            mz[i] = sp.magnetisation_direction[3 * p + 2];
+            k[i] = r.k_red[(size_t)lp];
+            al[i] = sp.damping[m];
+            dt[i] = r.dt;
+            sr[i] = r.sr[(size_t)lp];
+            sig[i] = r.sr[(size_t)lp] * std::sqrt(r.dt);
+            happ[i] = r.happ;
+            fred[i] = r.fapp;
+            fdt[i] = r.fapp * r.dt;
+            shape[i] = sp.field_shape[m];
+            msw[i] = sp.magnetisation[m];
+            key0[i] = (uint32_t)(uint64_t)sp.seed[m];
+            key1[i] = (uint32_t)((uint64_t)sp.seed[m] >> 32);
+            pid[i] = (uint32_t)lp;
+            shape_kind = shape_kind == -1 ? shape[i]
+                                          : (shape_kind == shape[i] ? shape_kind : (int)mp::kMixed);
+        }
+        d_ex.upload(cast_vec<T>(ex));
+        d_ey.upload(cast_vec<T>(ey));
+        d_ez.upload(cast_vec<T>(ez));
+        d_k.upload(cast_vec<T>(k));
+        d_al.upload(cast_vec<T>(al));
+        d_dt.upload(cast_vec<T>(dt));
+        d_sig.upload(cast_vec<T>(sig));
+        d_sr.upload(cast_vec<T>(sr));
+        d_happ.upload(cast_vec<T>(happ));
+        d_mx.upload(cast_vec<T>(mx));
+        d_my.upload(cast_vec<T>(my));
+        d_mz.upload(cast_vec<T>(mz));
+        d_fred.upload(fred);
+        d_fdt.upload(fdt);
+        d_dtd.upload(dt);
+        d_msw.upload(msw);
+        d_shape.upload(shape);
+        d_key0.upload(key0);
+        d_key1.upload(key1);
+        d_pid.upload(pid);
+        d_status.upload(std::vector<uint32_t>((size_t)n, 0));
+        d_sched.upload(sched);
+        d_it.upload(std::vector<unsigned long long>((size_t)n, 0));
+        d_fl.upload(std::vector<unsigned long long>((size_t)n, 0));
+        grid = (n + mp::kBlock - 1) / mp::kBlock;
+        n_waves = grid * (mp::kBlock / mp::kWave);
+        if (want_traj) d_samples.alloc((size_t)S * 3 * (size_t)n);
+        if (want_ens) {
+            // each recorded sample writes all of its (component, wave) slots;
+            // slots of samples taken at step 0 are replaced on the host
+            d_partial.alloc((size_t)S * 3 * (size_t)n_waves);
+            d_ens.alloc((size_t)S * 3);
+        }
+        if (array_noise) {
+            std::vector<double> nz((size_t)n_total * (size_t)n * 3);
+            for (int64_t st = 0; st < n_total; ++st)
+                for (int i = 0; i < n; ++i) {
+                    const int64_t p = plan.perm[(size_t)(b + i)];
+                    for (int d = 0; d < 3; ++d)
+                        nz[((size_t)st * n + i) * 3 + d] = sp.noise_array[((size_t)st * P + p) * 3 + d];
+                }
+            d_noise.upload(nz);
+        }
+        a.n = n;
+        a.n_waves = n_waves;
+        a.ex = d_ex.p; a.ey = d_ey.p; a.ez = d_ez.p;
+        a.k = d_k.p; a.alpha = d_al.p; a.dt = d_dt.p; a.sig = d_sig.p; a.sr = d_sr.p;
+        a.happ = d_happ.p; a.fred = d_fred.p; a.fdt = d_fdt.p; a.dtd = d_dtd.p;
+        a.shape = d_shape.p;
+        a.key0 = d_key0.p; a.key1 = d_key1.p; a.pid = d_pid.p;
+        a.msw = d_msw.p;
+        a.mx = d_mx.p; a.my = d_my.p; a.mz = d_mz.p;
+        a.status = d_status.p;
+        a.sched = d_sched.p;
+        a.S = S;
+        a.samples = want_traj ? d_samples.p : nullptr;
+        a.ens_partial = want_ens ? d_partial.p : nullptr;
+        a.tag = sp.noise_tag;
+        a.renorm = sp.renorm ? 1 : 0;
+        a.noise = array_noise ? d_noise.p : nullptr;
+    }
+
+    void enqueue(int64_t upto, hipStream_t s) override
+    {
+        upto = std::min(upto, n_total);
+        while (n_done < upto) {
+            const int64_t n1 = std::min(upto, n_done + spl);
+            while (s_next < S && sched[(size_t)s_next] < n_done + 1) ++s_next;
+            a.n0 = n_done;
+            a.n1 = n1;
+            a.s_next = s_next;
+            if (array_noise)
+                launch_shape<T, true>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s);
+            else
+                launch_shape<T, false>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s);
+            MP_HIP(hipGetLastError());
+            ++launches;
+            n_done = n1;
+        }
+    }
+
+    void state(std::vector<double> &out) override
+    {
+        std::vector<T> hx((size_t)n), hy((size_t)n), hz((size_t)n);
+        MP_HIP(hipMemcpy(hx.data(), d_mx.p, n * sizeof(T), hipMemcpyDeviceToHost));
+        MP_HIP(hipMemcpy(hy.data(), d_my.p, n * sizeof(T), hipMemcpyDeviceToHost));
+        MP_HIP(hipMemcpy(hz.data(), d_mz.p, n * sizeof(T), hipMemcpyDeviceToHost));
+        out.resize((size_t)n * 3);
+        for (int i = 0; i < n; ++i) {
+            out[(size_t)i * 3] = (double)hx[(size_t)i];
+            out[(size_t)i * 3 + 1] = (double)hy[(size_t)i];
+            out[(size_t)i * 3 + 2] = (double)hz[(size_t)i];
+        }
+    }
+
+    GroupOut collect(hipStream_t s) override
+    {
+        GroupOut o;
+        if (want_ens) {
+            hipLaunchKernelGGL(mp::ens_reduce_kernel, dim3((unsigned)(S * 3)), dim3(mp::kBlock), 0,
+                               s, d_partial.p, n_waves, S * 3, d_ens.p);
+            MP_HIP(hipGetLastError());
+        }
+        MP_HIP(hipStreamSynchronize(s));
+        state(o.state);
+        o.status.resize((size_t)n);
+        MP_HIP(hipMemcpy(o.status.data(), d_status.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> it((size_t)n), fl((size_t)n);
+        MP_HIP(hipMemcpy(it.data(), d_it.p, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        MP_HIP(hipMemcpy(fl.data(), d_fl.p, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            o.newton_iters += (int64_t)it[(size_t)i];
+            o.newton_fail += (int64_t)fl[(size_t)i];
+        }
+        if (want_traj) {
+            std::vector<T> hs(d_samples.n);
+            if (!hs.empty())
+                MP_HIP(hipMemcpy(hs.data(), d_samples.p, hs.size() * sizeof(T), hipMemcpyDeviceToHost));
+            o.samples.assign(hs.begin(), hs.end());
+        }
+        if (want_ens) {
+            o.ens.resize((size_t)S * 3);
+            MP_HIP(hipMemcpy(o.ens.data(), d_ens.p, o.ens.size() * sizeof(double),
+                             hipMemcpyDeviceToHost));
+        }
+        return o;
+    }
+};
+
+}  // namespace
+
+struct mp_session {
+    Plan plan;
+    HostCopy host;
+    std::vector<std::unique_ptr<GroupBase>> groups;
+    StreamGuard sg;
+    bool want_traj = false, want_ens = false;
+    int device = 0;
+    double kernel_ms = 0.0;
+};
+
+namespace simulation {
+
+ReducedMember reduce_member(const double *radius, const double *anisotropy, size_t n,
+                            double magnetisation, double damping, double temperature,
+                            double time_step, double end_time, double field_amplitude,
+                            double field_frequency)
+{
+#pragma clang fp contract(off)
+    ReducedMember r;
+    std::vector<double> volume(n);
+    for (size_t p = 0; p < n; ++p)
+        volume[p] = 4.0 / 3.0 * kPiHost * radius[p] * radius[p] * radius[p];
+    double vsum = 0.0;
+    for (size_t p = 0; p < n; ++p) vsum += volume[p];
+    r.average_volume = vsum / (double)n;
+    double ksum = 0.0;
+    for (size_t p = 0; p < n; ++p) ksum += anisotropy[p];
+    r.average_anisotropy = ksum / (double)n;
+    r.v_red.resize(n);
+    r.k_red.resize(n);
+    r.sr.resize(n);
+    for (size_t p = 0; p < n; ++p) {
+        r.v_red[p] = volume[p] / r.average_volume;
+        r.k_red[p] = anisotropy[p] / r.average_anisotropy;
+        r.sr[p] = std::sqrt(damping * kKB * temperature / (r.average_anisotropy * volume[p]) /
+                            (1 + damping * damping));
+    }
+    r.anisotropy_field = 2 * r.average_anisotropy / kMU0 / magnetisation;
+    r.time_factor = kGYROMAG * kMU0 * r.anisotropy_field / (1 + damping * damping);
+    r.dt = time_step * r.time_factor;
+    r.end_time = end_time * r.time_factor;
+    r.happ = field_amplitude / r.anisotropy_field;
+    r.fapp = field_frequency / r.time_factor;
+    return r;
+}
+
+int64_t sample_schedule(double dt, double end_time, int64_t max_samples, std::vector<int64_t> &k)
+{
+    // The reference's loop (lib/simulation.cpp:342-406) steps while
+    // t = step*dt <= s*Ts and records the state one step before it stopped;
+    // here the first such step is found by search instead of by stepping.
+    k.assign((size_t)max_samples, 0);
+    if (max_samples <= 1) return 0;
+    const double ts = end_time / (double)(max_samples - 1);
+    int64_t step = 0;
+    double t = 0.0;
+    for (int64_t s = 1; s < max_samples; ++s) {
+        const double x = (double)s * ts;
+        if (t <= x) {
+            double guess = std::floor(x / dt);
+            int64_t j = guess > 9.0e18 ? INT64_MAX / 2 : (int64_t)guess;
+            if (j < step + 1) j = step + 1;
+            while ((double)j * dt <= x) ++j;
+            while (j - 1 > step && (double)(j - 1) * dt > x) --j;
+            step = j;
+            t = (double)step * dt;
+        }
+        k[(size_t)s] = step - 1;
+    }
+    return step;
+}
+
+void validate(const EnsembleSpec &sp)
+{
+    require(sp.n_members >= 1, "n_members must be >= 1");
+    require(sp.member_offsets != nullptr, "member_offsets is required");
+    require(sp.member_offsets[0] == 0, "member_offsets[0] must be 0");
+    for (int64_t m = 0; m < sp.n_members; ++m)
+        require(sp.member_offsets[m + 1] > sp.member_offsets[m],
+                "every member needs at least one particle");
+    const int64_t P = sp.member_offsets[sp.n_members];
+    require(P <= (int64_t)INT32_MAX, "too many particles for one call");
+    require(sp.radius && sp.anisotropy && sp.anisotropy_axis && sp.magnetisation_direction &&
+                sp.location,
+            "particle arrays are required");
+    require(sp.magnetisation && sp.damping && sp.temperature && sp.field_shape &&
+                sp.field_amplitude && sp.field_frequency && sp.seed,
+            "member arrays are required");
+    require(sp.max_samples >= 1,
+            "max_samples must be >= 1 (-1 never terminates in the reference loop)");
+    require(sp.time_step > 0 && std::isfinite(sp.time_step), "time_step must be > 0");
+    require(sp.end_time >= 0 && std::isfinite(sp.end_time), "end_time must be >= 0");
+    require(sp.scheme == MP_SCHEME_HEUN || sp.scheme == MP_SCHEME_IMPLICIT_MIDPOINT,
+            "unknown scheme");
+    require(sp.precision == MP_PRECISION_FP32 || sp.precision == MP_PRECISION_FP64,
+            "unknown precision");
+    require(sp.noise == MP_NOISE_PHILOX || sp.noise == MP_NOISE_ARRAY, "unknown noise source");
+    require(sp.noise != MP_NOISE_ARRAY || sp.noise_array != nullptr,
+            "noise_array is required for MP_NOISE_ARRAY");
+    for (int64_t m = 0; m < sp.n_members; ++m) {
+        const int fs = sp.field_shape[m];
+        require(fs == MP_FIELD_SINE || fs == MP_FIELD_SQUARE || fs == MP_FIELD_CONSTANT,
+                "Must specify valid field::options enum");
+        require(sp.magnetisation[m] > 0 && sp.damping[m] >= 0 && sp.temperature[m] >= 0,
+                "magnetisation must be > 0, damping and temperature >= 0");
+    }
+    if (sp.interactions) {
+        for (int64_t m = 0; m < sp.n_members; ++m)
+            if (sp.member_offsets[m + 1] - sp.member_offsets[m] > 1)
+                throw Error(MP_ERR_UNSUPPORTED,
+                            "dipolar interactions between particles of one cluster are not "
+                            "built into this library yet; pass interactions=0");
+    }
+}
+
+}  // namespace simulation
+
+namespace {
+
+mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool ens)
+{
+    simulation::validate(sp);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw simulation::Error(MP_ERR_NODEV, "no HIP device visible");
+    require(sp.device >= 0 && sp.device < ndev, "device ordinal out of range");
+    MP_HIP(hipSetDevice(sp.device));
+
+    std::unique_ptr<mp_session> ss(new mp_session());
+    ss->device = sp.device;
+    ss->want_traj = traj;
+    ss->want_ens = ens;
+    Plan &plan = ss->plan;
+    const int64_t P = sp.member_offsets[sp.n_members];
+    HostCopy &h = ss->host;
+    h.M = sp.n_members;
+    h.P = P;
+    h.S = sp.max_samples;
+    h.m0.assign(sp.magnetisation_direction, sp.magnetisation_direction + 3 * P);
+    h.ms.assign(sp.magnetisation, sp.magnetisation + sp.n_members);
+    h.shape.assign(sp.field_shape, sp.field_shape + sp.n_members);
+
+    // reduce every member, schedule per member, group members by schedule
+    plan.red.reserve((size_t)sp.n_members);
+    plan.member_of.resize((size_t)P);
+    std::map<std::pair<double, double>, int> key_to_group;
+    std::vector<int> member_group((size_t)sp.n_members);
+    std::vector<std::pair<double, double>> group_keys;
+    for (int64_t m = 0; m < sp.n_members; ++m) {
+        const int64_t b = sp.member_offsets[m], e = sp.member_offsets[m + 1];
+        plan.red.push_back(simulation::reduce_member(
+            sp.radius + b, sp.anisotropy + b, (size_t)(e - b), sp.magnetisation[m], sp.damping[m],
+            sp.temperature[m], sp.time_step, sp.end_time, sp.field_amplitude[m],
+            sp.field_frequency[m]));
+        for (int64_t p = b; p < e; ++p) plan.member_of[(size_t)p] = m;
+        const auto key = std::make_pair(plan.red.back().dt, plan.red.back().end_time);
+        auto it = key_to_group.find(key);
+        if (it == key_to_group.end()) {
+            it = key_to_group.emplace(key, (int)group_keys.size()).first;
+            group_keys.push_back(key);
+        }
+        member_group[(size_t)m] = it->second;
+    }
+    const int G = (int)group_keys.size();
+    plan.group_begin.assign((size_t)G + 1, 0);
+    for (int g = 0; g < G; ++g) {
+        for (int64_t m = 0; m < sp.n_members; ++m)
+            if (member_group[(size_t)m] == g)
+                for (int64_t p = sp.member_offsets[m]; p < sp.member_offsets[m + 1]; ++p)
+                    plan.perm.push_back(p);
+        plan.group_begin[(size_t)g + 1] = (int64_t)plan.perm.size();
+        std::vector<int64_t> k;
+        const int64_t steps = simulation::sample_schedule(
+            group_keys[(size_t)g].first, group_keys[(size_t)g].second, sp.max_samples, k);
+        plan.sched.push_back(std::move(k));
+        plan.group_steps.push_back(steps);
+    }
+    if (sp.noise == MP_NOISE_ARRAY)
+        for (int g = 0; g < G; ++g)
+            require(plan.group_steps[(size_t)g] <= sp.noise_steps,
+                    "Exceeded allocated random number array size (noise_steps < steps)");
+    for (int g = 0; g < G; ++g) {
+        if (sp.precision == MP_PRECISION_FP32 && sp.scheme == MP_SCHEME_HEUN)
+            ss->groups.emplace_back(new GroupRun<float>(sp, plan, g, traj, ens));
+        else
+            ss->groups.emplace_back(new GroupRun<double>(sp, plan, g, traj, ens));
+    }
+    return ss.release();
+}
+
+void session_advance(mp_session *ss, int64_t n)
+{
+    MP_HIP(hipSetDevice(ss->device));
+    MP_HIP(hipEventRecord(ss->sg.e0, ss->sg.s));
+    for (auto &g : ss->groups) g->enqueue(g->n_done + n, ss->sg.s);
+    MP_HIP(hipEventRecord(ss->sg.e1, ss->sg.s));
+    MP_HIP(hipStreamSynchronize(ss->sg.s));
+    float ms = 0.f;
+    MP_HIP(hipEventElapsedTime(&ms, ss->sg.e0, ss->sg.e1));
+    ss->kernel_ms += ms;
+}
+
+void session_finish(mp_session *ss, mp_ensemble_out &out)
+{
+    int64_t remaining = 0;
+    for (auto &g : ss->groups) remaining = std::max(remaining, g->n_total - g->n_done);
+    if (remaining > 0) session_advance(ss, remaining);
+    MP_HIP(hipSetDevice(ss->device));
+    const HostCopy &h = ss->host;
+    const Plan &plan = ss->plan;
+    const int64_t P = h.P, S = h.S;
+    const bool want_traj = ss->want_traj && (out.mx || out.my || out.mz);
+    std::vector<double> state((size_t)P * 3), traj, ens;
+    std::vector<uint32_t> status((size_t)P, 0);
+    if (want_traj) traj.assign((size_t)S * 3 * (size_t)P, 0.0);
+    if (out.ens_m && ss->want_ens) ens.assign((size_t)S * 3, 0.0);
+    int64_t newton_iters = 0, newton_fail = 0;
+    for (size_t gi = 0; gi < ss->groups.size(); ++gi) {
+        GroupOut o = ss->groups[gi]->collect(ss->sg.s);
+        const int64_t b = plan.group_begin[gi], e = plan.group_begin[gi + 1];
+        const int n = (int)(e - b);
+        const std::vector<int64_t> &sched = plan.sched[gi];
+        for (int i = 0; i < n; ++i) {
+            const int64_t p = plan.perm[(size_t)(b + i)];
+            for (int d = 0; d < 3; ++d) state[(size_t)p * 3 + d] = o.state[(size_t)i * 3 + d];
+            status[(size_t)p] = o.status[(size_t)i];
+        }
+        newton_iters += o.newton_iters;
+        newton_fail += o.newton_fail;
+        // samples recorded at step 0 are the initial state, exactly as given
+        if (want_traj)
+            for (int64_t s = 0; s < S; ++s)
+                for (int i = 0; i < n; ++i) {
+                    const int64_t p = plan.perm[(size_t)(b + i)];
+                    for (int d = 0; d < 3; ++d)
+                        traj[((size_t)s * 3 + d) * (size_t)P + (size_t)p] =
+                            sched[(size_t)s] == 0
+                                ? h.m0[(size_t)p * 3 + d]
+                                : o.samples[((size_t)s * 3 + d) * (size_t)n + (size_t)i];
+                }
+        if (!ens.empty())
+            for (int64_t s = 0; s < S; ++s)
+                for (int d = 0; d < 3; ++d) {
+                    double v = o.ens[(size_t)s * 3 + d];
+                    if (sched[(size_t)s] == 0) {
+                        v = 0.0;
+                        for (int i = 0; i < n; ++i) {
+                            const int64_t p = plan.perm[(size_t)(b + i)];
+                            v += h.m0[(size_t)p * 3 + d] * h.ms[(size_t)plan.member_of[(size_t)p]];
+                        }
+                    }
+                    ens[(size_t)s * 3 + d] += v;
+                }
+    }
+
+    // ---- results in SI units (lib/simulation.cpp:610-621)
+    auto fill_time_field = [&](int64_t m, double *time, double *field) {
+        const simulation::ReducedMember &r = plan.red[(size_t)m];
+        const double ts = S > 1 ? r.end_time / (double)(S - 1) : 0.0;
+        for (int64_t s = 0; s < S; ++s) {
+            const double tr = s == 0 ? 0.0 : (double)s * ts;
+            if (time) time[s] = tr / r.time_factor;
+            if (field)
+                field[s] = applied_host(h.shape[(size_t)m], r.happ, r.fapp, tr) * r.anisotropy_field;
+        }
+    };
+    fill_time_field(0, out.time, out.field);
+    if (out.member_time || out.member_field)
+        for (int64_t m = 0; m < h.M; ++m)
+            fill_time_field(m, out.member_time ? out.member_time + m * S : nullptr,
+                            out.member_field ? out.member_field + m * S : nullptr);
+    if (want_traj) {
+        double *dst[3] = {out.mx, out.my, out.mz};
+        for (int64_t p = 0; p < P; ++p) {
+            const double ms = h.ms[(size_t)plan.member_of[(size_t)p]];
+            for (int d = 0; d < 3; ++d) {
+                if (!dst[d]) continue;
+                for (int64_t s = 0; s < S; ++s)
+                    dst[d][p * S + s] = traj[((size_t)s * 3 + d) * (size_t)P + (size_t)p] * ms;
+            }
+        }
+    }
+    if (out.ens_m && !ens.empty()) std::memcpy(out.ens_m, ens.data(), ens.size() * sizeof(double));
+    if (out.final_state) std::memcpy(out.final_state, state.data(), state.size() * sizeof(double));
+    if (out.status) std::memcpy(out.status, status.data(), status.size() * sizeof(uint32_t));
+    out.n_steps = plan.group_steps[0];
+    out.newton_iters = newton_iters;
+    out.newton_failures = newton_fail;
+    out.kernel_ms = ss->kernel_ms;
+}
+
+int fail(const simulation::Error &e)
+{
+    g_last_error = e.what();
+    return e.code;
+}
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return MP_OK;
+    } catch (const simulation::Error &e) {
+        return fail(e);
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return MP_ERR_ALLOC;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return MP_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+namespace simulation {
+
+void ensemble_run(const EnsembleSpec &sp, EnsembleResult &out)
+{
+    std::unique_ptr<mp_session> ss(
+        session_create(sp, out.mx || out.my || out.mz, out.ens_m != nullptr));
+    session_finish(ss.get(), out);
+}
+
+}  // namespace simulation
+
+// ====================================================================== C ABI
+extern "C" {
+
+int mp_ensemble_run(const mp_ensemble_spec *spec, mp_ensemble_out *out)
+{
+    return guarded([&] {
+        if (!spec || !out) throw simulation::Error(MP_ERR_ARG, "spec and out are required");
+        simulation::ensemble_run(*spec, *out);
+    });
+}
+
+int mp_full_dynamics(const double *radius, const double *anisotropy,
+                     const double *anisotropy_axis, const double *magnetisation_direction,
+                     const double *location, size_t n_particles, double magnetisation,
+                     double damping, double temperature, int renorm, int interactions,
+                     int use_implicit, double eps, double time_step, double end_time,
+                     size_t max_samples, long seed, int field_shape, double field_amplitude,
+                     double field_frequency, int precision, double *time, double *field,
+                     double *mx, double *my, double *mz)
+{
+    int64_t offs[2] = {0, (int64_t)n_particles};
+    int32_t shape = field_shape;
+    int64_t sd = seed;
+    mp_ensemble_spec sp{};
+    sp.n_members = 1;
+    sp.member_offsets = offs;
+    sp.radius = radius;
+    sp.anisotropy = anisotropy;
+    sp.anisotropy_axis = anisotropy_axis;
+    sp.magnetisation_direction = magnetisation_direction;
+    sp.location = location;
+    sp.magnetisation = &magnetisation;
+    sp.damping = &damping;
+    sp.temperature = &temperature;
+    sp.field_shape = &shape;
+    sp.field_amplitude = &field_amplitude;
+    sp.field_frequency = &field_frequency;
+    sp.seed = &sd;
+    sp.time_step = time_step;
+    sp.end_time = end_time;
+    sp.max_samples = (int64_t)max_samples;
+    sp.renorm = renorm;
+    sp.interactions = interactions;
+    sp.scheme = use_implicit ? MP_SCHEME_IMPLICIT_MIDPOINT : MP_SCHEME_HEUN;
+    sp.precision = precision;
+    sp.implicit_tol = eps;
+    sp.implicit_max_iter = 1000;
+    sp.noise = MP_NOISE_PHILOX;
+    mp_ensemble_out o{};
+    o.time = time;
+    o.field = field;
+    o.mx = mx;
+    o.my = my;
+    o.mz = mz;
+    return mp_ensemble_run(&sp, &o);
+}
+
+int mp_session_create(const mp_ensemble_spec *spec, int want_trajectories, int want_ensemble,
+                      mp_session **session)
+{
+    return guarded([&] {
+        if (!spec || !session) throw simulation::Error(MP_ERR_ARG, "spec and session are required");
+        *session = session_create(*spec, want_trajectories != 0, want_ensemble != 0);
+    });
+}
+
+int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done)
+{
+    return guarded([&] {
+        if (!session || n_steps < 0) throw simulation::Error(MP_ERR_ARG, "bad session arguments");
+        session_advance(session, n_steps);
+        if (steps_done) *steps_done = session->groups[0]->n_done;
+    });
+}
+
+int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
+                    double *kernel_ms)
+{
+    return guarded([&] {
+        if (!session) throw simulation::Error(MP_ERR_ARG, "session is required");
+        int64_t tot = 0, l = 0;
+        for (auto &g : session->groups) {
+            tot = std::max(tot, g->n_total);
+            l += g->launches;
+        }
+        if (n_steps_total) *n_steps_total = tot;
+        if (launches) *launches = l;
+        if (kernel_ms) *kernel_ms = session->kernel_ms;
+    });
+}
+
+int mp_session_state(mp_session *session, double *state, int64_t *step)
+{
+    return guarded([&] {
+        if (!session || !state) throw simulation::Error(MP_ERR_ARG, "session and state are required");
+        MP_HIP(hipSetDevice(session->device));
+        MP_HIP(hipStreamSynchronize(session->sg.s));
+        const Plan &plan = session->plan;
+        for (size_t gi = 0; gi < session->groups.size(); ++gi) {
+            std::vector<double> st;
+            session->groups[gi]->state(st);
+            const int64_t b = plan.group_begin[gi], e = plan.group_begin[gi + 1];
+            for (int64_t i = 0; i < e - b; ++i) {
+                const int64_t p = plan.perm[(size_t)(b + i)];
+                for (int d = 0; d < 3; ++d) state[p * 3 + d] = st[(size_t)i * 3 + d];
+            }
+        }
+        if (step) *step = session->groups[0]->n_done;
+    });
+}
+
+int mp_session_finish(mp_session *session, mp_ensemble_out *out)
+{
+    return guarded([&] {
+        if (!session || !out) throw simulation::Error(MP_ERR_ARG, "session and out are required");
+        session_finish(session, *out);
+    });
+}
+
+void mp_session_destroy(mp_session *session) { delete session; }
+
+int mp_sample_schedule(double dt_reduced, double end_reduced, int64_t max_samples, int64_t *k,
+                       int64_t *n_steps)
+{
+    return guarded([&] {
+        if (max_samples < 1 || !k || !n_steps || !(dt_reduced > 0))
+            throw simulation::Error(MP_ERR_ARG, "invalid schedule arguments");
+        std::vector<int64_t> v;
+        *n_steps = simulation::sample_schedule(dt_reduced, end_reduced, max_samples, v);
+        std::memcpy(k, v.data(), v.size() * sizeof(int64_t));
+    });
+}
+
+int mp_wiener_normals(uint64_t seed, uint32_t particle, uint32_t tag, uint64_t j0, size_t n,
+                      int precision, double *out, int device)
+{
+    return guarded([&] {
+        if (!out) throw simulation::Error(MP_ERR_ARG, "out is required");
+        if (n == 0) return;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw simulation::Error(MP_ERR_NODEV, "no HIP device visible");
+        if (device < 0 || device >= ndev) throw simulation::Error(MP_ERR_ARG, "bad device");
+        MP_HIP(hipSetDevice(device));
+        DevBuf<double> d(n);
+        const unsigned grid = (unsigned)((n + 255) / 256);
+        if (precision == MP_PRECISION_FP32)
+            hipLaunchKernelGGL(mp::wiener_kernel<float>, dim3(grid), dim3(256), 0, 0,
+                               (uint32_t)seed, (uint32_t)(seed >> 32), particle, tag, j0,
+                               (int64_t)n, d.p);
+        else
+            hipLaunchKernelGGL(mp::wiener_kernel<double>, dim3(grid), dim3(256), 0, 0,
+                               (uint32_t)seed, (uint32_t)(seed >> 32), particle, tag, j0,
+                               (int64_t)n, d.p);
+        MP_HIP(hipGetLastError());
+        MP_HIP(hipMemcpy(out, d.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+void mp_constants(double *kb, double *mu0, double *gamma)
+{
+    if (kb) *kb = kKB;
+    if (mu0) *mu0 = kMU0;
+    if (gamma) *gamma = kGYROMAG;
+}
+
+const char *mp_last_error(void) { return g_last_error.c_str(); }
+
+const char *mp_version(void) { return "magpy_amd 0.1 (abi 1, gfx950)"; }
+
+int mp_device_info(int device, char *buf, size_t len)
+{
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        g_last_error = hipGetErrorString(e);
+        return MP_ERR_NODEV;
+    }
+    if (buf && len)
+        std::snprintf(buf, len, "%s (%s), %d CUs, %d MHz, %.1f GiB", prop.name, prop.gcnArchName,
+                      prop.multiProcessorCount, prop.clockRate / 1000,
+                      (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
+    return MP_OK;
+}
+
+}  // extern "C"

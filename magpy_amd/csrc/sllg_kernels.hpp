@@ -1,0 +1,567 @@
+// sllg_kernels.hpp -- gfx950 kernels of the stochastic LLG ensemble integrator.
+//
+// Independent particles (clusters of one, or interactions off): one lane per
+// particle, the state and all parameters in registers for a whole step block,
+// Wiener increments generated in-lane from the Philox stream (philox.h), and
+// samples written SoA ([sample][component][particle]) so a wave's stores are
+// contiguous.  Ensemble sums are reduced per wave in FP64 and written as
+// per-wave partials, summed afterwards in a fixed order (deterministic).
+//
+// Physics restated from owlas/magpy:
+//   drift     a(m,h) = -m x h + alpha (h |m|^2 - m (m.h))   lib/llg.cpp:14-29
+//   diffusion B(m) w = sr [-m x w + alpha (w |m|^2 - m (m.w))] lib/llg.cpp:92-106
+//   field     h = k (m.e) e + h_app(t) z                   lib/field.cpp:116-132, 81-88
+//   Heun      lib/integrators.cpp:372-405, called with t = (n+1) dt for the
+//             step n -> n+1 (lib/simulation.cpp:350-375)
+// Both a and B w are the same bilinear map L(m, v) = -m x v + alpha (v |m|^2 -
+// m (m.v)) applied to v = h and v = sr w, so one stage costs one L:
+//   a dt + B w sqrt(dt) = L(m, h dt + sr sqrt(dt) w).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "philox.h"
+
+namespace mp {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+enum FieldKind { kSine = 0, kSquare = 1, kConstant = 2, kMixed = 3 };
+
+// ------------------------------------------------------------------ numerics
+template <typename T>
+struct Num;
+
+template <>
+struct Num<float> {
+    // Native transcendentals: v_log_f32 (log2), v_sqrt_f32, v_sin/v_cos_f32
+    // (argument in revolutions, so sin(2 pi v) is one instruction).
+    static __device__ inline void box_muller(uint32_t a, uint32_t b, float &z0, float &z1)
+    {
+        const float u = fmaf((float)a, 0x1p-32f, 0x1p-33f);
+        const float v = fmaf((float)b, 0x1p-32f, 0x1p-33f);
+        const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061883f * __builtin_amdgcn_logf(u));
+        z0 = r * __builtin_amdgcn_cosf(v);
+        z1 = r * __builtin_amdgcn_sinf(v);
+    }
+    // h sin(2 pi c) for c = f t in cycles, reduced to [0,1) in FP64 first.
+    static __device__ inline float sine_cycles(double c)
+    {
+        return __builtin_amdgcn_sinf((float)(c - floor(c)));
+    }
+};
+
+template <>
+struct Num<double> {
+    static __device__ inline void box_muller(uint32_t a, uint32_t b, double &z0, double &z1)
+    {
+        const double u = ((double)a + 0.5) * 0x1p-32;
+        const double v = ((double)b + 0.5) * 0x1p-32;
+        const double r = sqrt(-2.0 * log(u));
+        const double th = 2.0 * kPi * v;
+        double s, c;
+        sincos(th, &s, &c);
+        z0 = r * c;
+        z1 = r * s;
+    }
+};
+
+// Applied field along z at stage time.  FP32 evaluates the phase from cycles
+// c = f_red * t in FP64; FP64 evaluates h sin(2 pi f t) with t = (n+1) dt as
+// the reference does (lib/field.cpp:37-54).
+template <typename T, int kShape>
+struct Applied;
+
+template <int kShape>
+struct Applied<float, kShape> {
+    static __device__ inline float at(int shape, float amp, double cycles)
+    {
+        const int s = kShape == kMixed ? shape : kShape;
+        if (s == kConstant) return amp;
+        if (s == kSine) return amp * Num<float>::sine_cycles(cycles);
+        return (cycles - floor(cycles)) >= 0.5 ? -amp : amp;  // (int)(2 f t) % 2
+    }
+};
+
+template <int kShape>
+struct Applied<double, kShape> {
+    static __device__ inline double at(int shape, double amp, double f, double t)
+    {
+        const int s = kShape == kMixed ? shape : kShape;
+        if (s == kConstant) return amp;
+        if (s == kSine) return amp * sin(2 * kPi * f * t);
+        return amp * ((int)(t * f * 2) % 2 ? -1 : 1);
+    }
+};
+
+// L(m, v) = -m x v + alpha (v |m|^2 - m (m.v))
+template <typename T>
+__device__ inline void llg_map(T mx, T my, T mz, T vx, T vy, T vz, T al, T &lx, T &ly, T &lz)
+{
+    const T cx = my * vz - mz * vy;
+    const T cy = mz * vx - mx * vz;
+    const T cz = mx * vy - my * vx;
+    const T d = mx * vx + my * vy + mz * vz;
+    const T q = mx * mx + my * my + mz * mz;
+    lx = al * (vx * q - mx * d) - cx;
+    ly = al * (vy * q - my * d) - cy;
+    lz = al * (vz * q - mz * d) - cz;
+}
+
+// ------------------------------------------------------------------ arguments
+template <typename T>
+struct IndepArgs {
+    int n;          // particles
+    int n_waves;    // waves in the grid (ensemble partial stride)
+    // per-particle parameters (member values broadcast to its particles)
+    const T *ex, *ey, *ez;  // easy axis
+    const T *k;             // K_i / K_avg
+    const T *alpha;
+    const T *dt;            // reduced step
+    const T *sig;           // sr * sqrt(dt)
+    const T *sr;            // reduced thermal field strength
+    const T *happ;          // reduced field amplitude
+    const double *fred;     // reduced field frequency
+    const double *fdt;      // f_red * dt_red, cycles per step
+    const double *dtd;      // reduced step in FP64 (field time)
+    const int *shape;
+    const uint32_t *key0, *key1, *pid;
+    const double *msw;      // Ms of the particle's member (ensemble sums)
+    T *mx, *my, *mz;        // state in/out
+    uint32_t *status;
+    // sampling
+    const int64_t *sched;   // [S] step index recorded as sample s
+    int64_t S;
+    int64_t s_next;         // first sample with sched[s] >= n0 + 1
+    T *samples;             // [S][3][n] or null
+    double *ens_partial;    // [S][3][n_waves] or null
+    // step block [n0, n1)
+    int64_t n0, n1;
+    uint32_t tag;
+    int renorm;
+    const double *noise;    // [steps][n][3] (array noise) or null
+};
+
+template <typename T>
+__device__ inline void wave_sum3(double &x, double &y, double &z)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, kWave);
+        y += __shfl_xor(y, off, kWave);
+        z += __shfl_xor(z, off, kWave);
+    }
+}
+
+template <typename T>
+__device__ inline void record_sample(const IndepArgs<T> &a, int64_t s, int p, bool live, T mx,
+                                     T my, T mz, double w)
+{
+    if (a.samples != nullptr && live) {
+        T *base = a.samples + (size_t)s * 3 * (size_t)a.n;
+        base[p] = mx;
+        base[(size_t)a.n + p] = my;
+        base[2 * (size_t)a.n + p] = mz;
+    }
+    if (a.ens_partial != nullptr) {
+        double x = live ? (double)mx * w : 0.0;
+        double y = live ? (double)my * w : 0.0;
+        double z = live ? (double)mz * w : 0.0;
+        wave_sum3<T>(x, y, z);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+            double *dst = a.ens_partial + (size_t)s * 3 * (size_t)a.n_waves;
+            dst[wave] = x;
+            dst[(size_t)a.n_waves + wave] = y;
+            dst[2 * (size_t)a.n_waves + wave] = z;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Heun
+// One block of steps [n0, n1) for independent particles.  Steps are walked in
+// groups of four, whose twelve normals come from three Philox calls.
+template <typename T, int kShape, bool kArrayNoise>
+__global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = p < a.n;
+    const int q = live ? p : a.n - 1;
+
+    T mx = a.mx[q], my = a.my[q], mz = a.mz[q];
+    const T ex = a.ex[q], ey = a.ey[q], ez = a.ez[q];
+    const T k = a.k[q], al = a.alpha[q], dt = a.dt[q], sig = a.sig[q], amp = a.happ[q];
+    const double fred = a.fred[q], fdt = a.fdt[q], dtd = a.dtd[q];
+    const int shape = a.shape[q];
+    const double msw = a.msw[q];
+    PhiloxKeys keys;
+    keys.init(a.key0[q], a.key1[q]);
+    const uint32_t pid = a.pid[q];
+
+    int64_t s = a.s_next;
+    int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+
+    const int64_t g0 = a.n0 >> 2, g1 = (a.n1 + 3) >> 2;
+    for (int64_t g = g0; g < g1; ++g) {
+        T z[12];
+        if (!kArrayNoise) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const uint64_t c = (uint64_t)(3 * g + i);
+                const u32x4 w = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, a.tag}, keys);
+                Num<T>::box_muller(w.x, w.y, z[4 * i], z[4 * i + 1]);
+                Num<T>::box_muller(w.z, w.w, z[4 * i + 2], z[4 * i + 3]);
+            }
+        }
+        const double base = (double)(4 * g);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t n = 4 * g + u;
+            if (n < a.n0 || n >= a.n1) continue;
+            T w0, w1, w2;
+            if (kArrayNoise) {
+                const double *src = a.noise + ((size_t)n * (size_t)a.n + (size_t)q) * 3;
+                w0 = (T)src[0];
+                w1 = (T)src[1];
+                w2 = (T)src[2];
+            } else {
+                w0 = z[3 * u];
+                w1 = z[3 * u + 1];
+                w2 = z[3 * u + 2];
+            }
+            // field times: t1 = (n+1) dt, t2 = t1 + dt
+            T h1, h2;
+            if constexpr (sizeof(T) == 4) {
+                const double c1 = (base + (double)(u + 1)) * fdt;
+                h1 = Applied<T, kShape>::at(shape, amp, c1);
+                h2 = Applied<T, kShape>::at(shape, amp, c1 + fdt);
+            } else {
+                const double t1 = (base + (double)(u + 1)) * dtd;
+                h1 = Applied<T, kShape>::at(shape, amp, fred, t1);
+                h2 = Applied<T, kShape>::at(shape, amp, fred, t1 + dtd);
+            }
+            const T sx = sig * w0, sy = sig * w1, sz = sig * w2;
+            // stage 1: predictor
+            T d = (mx * ex + my * ey + mz * ez) * k;
+            T vx = (d * ex) * dt + sx, vy = (d * ey) * dt + sy, vz = (d * ez + h1) * dt + sz;
+            T ax, ay, az;
+            llg_map(mx, my, mz, vx, vy, vz, al, ax, ay, az);
+            const T tx = mx + ax, ty = my + ay, tz = mz + az;
+            // stage 2: corrector
+            d = (tx * ex + ty * ey + tz * ez) * k;
+            vx = (d * ex) * dt + sx;
+            vy = (d * ey) * dt + sy;
+            vz = (d * ez + h2) * dt + sz;
+            T bx, by, bz;
+            llg_map(tx, ty, tz, vx, vy, vz, al, bx, by, bz);
+            mx += T(0.5) * (ax + bx);
+            my += T(0.5) * (ay + by);
+            mz += T(0.5) * (az + bz);
+            if (a.renorm) {  // lib/simulation.cpp:379-387
+                const T nrm = sqrt(mx * mx + my * my + mz * mz);
+                mx = mx / nrm;
+                my = my / nrm;
+                mz = mz / nrm;
+            }
+            if (n + 1 == next_rec) {
+                do {
+                    record_sample(a, s, p, live, mx, my, mz, msw);
+                    ++s;
+                    next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+                } while (next_rec == n + 1);
+            }
+        }
+    }
+    if (live) {
+        a.mx[p] = mx;
+        a.my[p] = my;
+        a.mz[p] = mz;
+        if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[p] |= 1u;
+    }
+}
+
+// ------------------------------------------------------------------ implicit midpoint
+// FP64, one lane per particle.  Restates integrator::implicit_midpoint
+// (lib/integrators.cpp:576-651) with optimisation::newton_raphson_noinv
+// (lib/optimisation.cpp:81-149) and the LLG Jacobians of lib/llg.cpp:68-81,
+// 118-158, including the reference's Jacobian as it is written (drift term
+// without dt; two diffusion-Jacobian entries taken at m2): they steer the
+// Newton path, not the root it converges to.
+struct ImplicitArgs {
+    IndepArgs<double> base;
+    double eps;
+    int max_iter;
+    unsigned long long *newton_iters;   // per-lane accumulators [n]
+    unsigned long long *newton_fail;    // [n]
+};
+
+__device__ inline void drift3(const double *m, double al, const double *h, double *d)
+{
+    d[0] = m[2] * h[1] - m[1] * h[2] + al * (h[0] * (m[1] * m[1] + m[2] * m[2]) - m[0] * (m[1] * h[1] + m[2] * h[2]));
+    d[1] = m[0] * h[2] - m[2] * h[0] + al * (h[1] * (m[0] * m[0] + m[2] * m[2]) - m[1] * (m[0] * h[0] + m[2] * h[2]));
+    d[2] = m[1] * h[0] - m[0] * h[1] + al * (h[2] * (m[0] * m[0] + m[1] * m[1]) - m[2] * (m[0] * h[0] + m[1] * h[1]));
+}
+
+__device__ inline void diffusion3(const double *m, double sr, double al, double *B)
+{
+    B[0] = al * sr * (m[1] * m[1] + m[2] * m[2]);
+    B[1] = sr * (m[2] - al * m[0] * m[1]);
+    B[2] = -sr * (m[1] + al * m[0] * m[2]);
+    B[3] = -sr * (m[2] + al * m[0] * m[1]);
+    B[4] = al * sr * (m[0] * m[0] + m[2] * m[2]);
+    B[5] = sr * (m[0] - al * m[1] * m[2]);
+    B[6] = sr * (m[1] - al * m[0] * m[2]);
+    B[7] = -sr * (m[0] + al * m[1] * m[2]);
+    B[8] = al * sr * (m[0] * m[0] + m[1] * m[1]);
+}
+
+// d(drift)/dm, row-major, hj = field Jacobian (row-major)
+__device__ inline void drift_jac3(const double *m, double a, const double *h, const double *hj, double *j)
+{
+    j[0] = m[2] * hj[3] - m[1] * hj[6] + a * (-m[1] * h[1] - m[2] * h[2] + (m[1] * m[1] + m[2] * m[2]) * hj[0] - m[0] * (m[1] * hj[3] + m[2] * hj[6]));
+    j[1] = -h[2] + m[2] * hj[4] - m[1] * hj[7] + a * (2 * m[1] * h[0] + (m[1] * m[1] + m[2] * m[2]) * hj[1] - m[0] * (h[1] + m[1] * hj[4] + m[2] * hj[7]));
+    j[2] = h[1] + m[2] * hj[5] - m[1] * hj[8] + a * (2 * m[2] * h[0] + (m[1] * m[1] + m[2] * m[2]) * hj[2] - m[0] * (h[2] + m[1] * hj[5] + m[2] * hj[8]));
+    j[3] = h[2] - m[2] * hj[0] + m[0] * hj[6] + a * (2 * m[0] * h[1] + (m[0] * m[0] + m[2] * m[2]) * hj[3] - m[1] * (h[0] + m[0] * hj[0] + m[2] * hj[6]));
+    j[4] = -m[2] * hj[1] + m[0] * hj[7] + a * (-m[0] * h[0] - m[2] * h[2] + (m[0] * m[0] + m[2] * m[2]) * hj[4] - m[1] * (m[0] * hj[1] + m[2] * hj[7]));
+    j[5] = -h[0] - m[2] * hj[2] + m[0] * hj[8] + a * (2 * m[2] * h[1] + (m[0] * m[0] + m[2] * m[2]) * hj[5] - m[1] * (h[2] + m[0] * hj[2] + m[2] * hj[8]));
+    j[6] = -h[1] + m[1] * hj[0] - m[0] * hj[3] + a * (2 * m[0] * h[2] + (m[0] * m[0] + m[1] * m[1]) * hj[6] - m[2] * (h[0] + m[0] * hj[0] + m[1] * hj[3]));
+    j[7] = h[0] + m[1] * hj[1] - m[0] * hj[4] + a * (2 * m[1] * h[2] + (m[0] * m[0] + m[1] * m[1]) * hj[7] - m[2] * (h[1] + m[0] * hj[1] + m[1] * hj[4]));
+    j[8] = m[1] * hj[2] - m[0] * hj[5] + a * (-m[0] * h[0] - m[1] * h[1] + (m[0] * m[0] + m[1] * m[1]) * hj[8] - m[2] * (m[0] * hj[2] + m[1] * hj[5]));
+}
+
+// jb[z + 3y + 9x] = dB[x][y]/dm[z] as lib/llg.cpp:118-158 writes it
+__device__ inline void diffusion_jac3(const double *m, double sr, double al, double *jb)
+{
+    jb[0] = 0;                  jb[1] = 2 * al * sr * m[1];  jb[2] = 2 * al * sr * m[2];
+    jb[3] = -al * sr * m[1];    jb[4] = -al * sr * m[2];     jb[5] = sr;
+    jb[6] = -al * sr * m[2];    jb[7] = -sr;                 jb[8] = -al * sr * m[0];
+    jb[9] = -al * sr * m[1];    jb[10] = -al * sr * m[0];    jb[11] = -sr;
+    jb[12] = 2 * al * sr * m[0]; jb[13] = 0;                 jb[14] = 2 * al * sr * m[2];
+    jb[15] = sr;                jb[16] = -al * sr * m[2];    jb[17] = -al * sr * m[1];
+    jb[18] = -al * sr * m[2];   jb[19] = sr;                 jb[20] = -al * sr * m[0];
+    jb[21] = -sr;               jb[22] = -al * sr * m[2];    jb[23] = -al * sr * m[1];
+    jb[24] = 2 * al * sr * m[0]; jb[25] = 2 * al * sr * m[2]; jb[26] = 0;
+}
+
+// LU with partial pivoting (dgetf2 order: reciprocal scaling) and solve of
+// J x = b in place.  Returns false on an exactly zero pivot.
+__device__ inline bool solve3(double *A, double *b)
+{
+    int piv[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        int p = j;
+        double best = fabs(A[j * 3 + j]);
+#pragma unroll
+        for (int i = j + 1; i < 3; ++i)
+            if (fabs(A[i * 3 + j]) > best) { best = fabs(A[i * 3 + j]); p = i; }
+        piv[j] = p;
+        if (A[p * 3 + j] == 0.0) return false;
+        if (p != j) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { const double t = A[j * 3 + c]; A[j * 3 + c] = A[p * 3 + c]; A[p * 3 + c] = t; }
+        }
+        const double rcp = 1.0 / A[j * 3 + j];
+#pragma unroll
+        for (int i = j + 1; i < 3; ++i) A[i * 3 + j] *= rcp;
+#pragma unroll
+        for (int i = j + 1; i < 3; ++i)
+#pragma unroll
+            for (int c = j + 1; c < 3; ++c) A[i * 3 + c] -= A[i * 3 + j] * A[j * 3 + c];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        if (piv[j] != j) { const double t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
+    b[1] -= A[3] * b[0];
+    b[2] -= A[6] * b[0] + A[7] * b[1];
+    b[2] /= A[8];
+    b[1] = (b[1] - A[5] * b[2]) / A[4];
+    b[0] = (b[0] - A[1] * b[1] - A[2] * b[2]) / A[0];
+    return true;
+}
+
+template <int kShape, bool kArrayNoise>
+__global__ __launch_bounds__(kBlock) void implicit_indep_kernel(ImplicitArgs ia)
+{
+    const IndepArgs<double> &a = ia.base;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = p < a.n;
+    const int q = live ? p : a.n - 1;
+
+    double x0[3] = {a.mx[q], a.my[q], a.mz[q]};
+    const double e[3] = {a.ex[q], a.ey[q], a.ez[q]};
+    const double k = a.k[q], al = a.alpha[q], dt = a.dt[q], amp = a.happ[q];
+    const double sr = a.sr[q];
+    const double fred = a.fred[q];
+    const int shape = a.shape[q];
+    const double msw = a.msw[q];
+    PhiloxKeys keys;
+    keys.init(a.key0[q], a.key1[q]);
+    const uint32_t pid = a.pid[q];
+    const double ah = sqrt(2 * 1000.0 * fabs(log(dt)));
+    const double sdt = sqrt(dt);
+    // field Jacobian k e e^T (lib/field.cpp:159-174)
+    double hj[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) hj[i * 3 + j] = k * e[i] * e[j];
+
+    unsigned long long iters = 0, fails = 0;
+    int64_t s = a.s_next;
+    int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+    double zbuf[4];
+    uint64_t have = UINT64_MAX;
+
+    for (int64_t n = a.n0; n < a.n1; ++n) {
+        double w[3];
+        if (kArrayNoise) {
+            const double *src = a.noise + ((size_t)n * (size_t)a.n + (size_t)q) * 3;
+            w[0] = src[0]; w[1] = src[1]; w[2] = src[2];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const uint64_t j = 3 * (uint64_t)n + d;
+                const uint64_t c = j >> 2;
+                if (c != have) {
+                    const u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, a.tag}, keys);
+                    Num<double>::box_muller(r.x, r.y, zbuf[0], zbuf[1]);
+                    Num<double>::box_muller(r.z, r.w, zbuf[2], zbuf[3]);
+                    have = c;
+                }
+                const int jj = (int)(j & 3);
+                w[d] = jj == 0 ? zbuf[0] : jj == 1 ? zbuf[1] : jj == 2 ? zbuf[2] : zbuf[3];
+            }
+        }
+        const double t = (double)(n + 1) * dt;
+        double dwm[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dwm[d] = fmax(-ah, fmin(ah, w[d])) * sdt;
+
+        // Euler guess from sde(x0, t) (lib/integrators.cpp:605-614)
+        double h[3], dr[3], B[9];
+        {
+            const double dot = (x0[0] * e[0] + x0[1] * e[1] + x0[2] * e[2]) * k;
+            h[0] = dot * e[0];
+            h[1] = dot * e[1];
+            h[2] = dot * e[2] + Applied<double, kShape>::at(shape, amp, fred, t);
+        }
+        drift3(x0, al, h, dr);
+        diffusion3(x0, sr, al, B);
+        double y[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double g = dr[i] * dt;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) g += B[i * 3 + j] * dwm[j];
+            y[i] = (g + x0[i]) / 2;
+        }
+        const double happ_mid = Applied<double, kShape>::at(shape, amp, fred, t + dt / 2);
+        const double tol = ia.eps * sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+        double err = 2 * tol;
+        int iter = ia.max_iter;
+        bool singular = false;
+        while ((err > tol) && (iter-- > 0)) {
+            const double xt[3] = {y[0], y[1], y[2]};
+            const double dot = (xt[0] * e[0] + xt[1] * e[1] + xt[2] * e[2]) * k;
+            h[0] = dot * e[0];
+            h[1] = dot * e[1];
+            h[2] = dot * e[2] + happ_mid;
+            drift3(xt, al, h, dr);
+            diffusion3(xt, sr, al, B);
+            double J[9], jb[27], F[3];
+            drift_jac3(xt, al, h, hj, J);
+            diffusion_jac3(xt, sr, al, jb);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                F[i] = xt[i] - 0.5 * dr[i] * dt - x0[i];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) F[i] -= 0.5 * B[i * 3 + j] * dwm[j];
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double v = (double)(i == j) - 0.5 * J[i * 3 + j];
+#pragma unroll
+                    for (int kk = 0; kk < 3; ++kk) v -= 0.5 * jb[i * 9 + kk * 3 + j] * dwm[kk];
+                    J[i * 3 + j] = v;
+                }
+            double dlt[3] = {-F[0], -F[1], -F[2]};
+            ++iters;
+            if (!solve3(J, dlt)) {  // LAPACK_ERR: x_root left at -F
+                y[0] = dlt[0]; y[1] = dlt[1]; y[2] = dlt[2];
+                singular = true;
+                break;
+            }
+            err = sqrt(dlt[0] * dlt[0] + dlt[1] * dlt[1] + dlt[2] * dlt[2]);
+            y[0] = dlt[0] + xt[0];
+            y[1] = dlt[1] + xt[1];
+            y[2] = dlt[2] + xt[2];
+        }
+        if (singular || iter == -1) ++fails;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) x0[i] = 2 * y[i] - x0[i];
+        if (a.renorm) {
+            const double nrm = sqrt(x0[0] * x0[0] + x0[1] * x0[1] + x0[2] * x0[2]);
+            x0[0] = x0[0] / nrm; x0[1] = x0[1] / nrm; x0[2] = x0[2] / nrm;
+        }
+        if (n + 1 == next_rec) {
+            do {
+                record_sample(a, s, p, live, x0[0], x0[1], x0[2], msw);
+                ++s;
+                next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+            } while (next_rec == n + 1);
+        }
+    }
+    if (live) {
+        a.mx[p] = x0[0];
+        a.my[p] = x0[1];
+        a.mz[p] = x0[2];
+        ia.newton_iters[p] += iters;
+        ia.newton_fail[p] += fails;
+        uint32_t st = 0;
+        if (!(isfinite(x0[0]) && isfinite(x0[1]) && isfinite(x0[2]))) st |= 1u;
+        if (fails) st |= 2u;
+        a.status[p] |= st;
+    }
+}
+
+// Sum per-wave partials of each (sample, component) in a fixed order.
+__global__ void ens_reduce_kernel(const double *partial, int n_waves, int64_t rows, double *out)
+{
+    const int64_t r = blockIdx.x;
+    if (r >= rows) return;
+    __shared__ double buf[kBlock];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n_waves; i += blockDim.x) acc += partial[r * n_waves + i];
+    buf[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) buf[threadIdx.x] += buf[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[r] = buf[0];
+}
+
+// normals j0 .. j0+n-1 of one particle's stream, one lane per normal
+template <typename T>
+__global__ void wiener_kernel(uint32_t key0, uint32_t key1, uint32_t pid, uint32_t tag, uint64_t j0,
+                              int64_t n, double *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t j = j0 + (uint64_t)i;
+    const uint64_t c = j >> 2;
+    PhiloxKeys keys;
+    keys.init(key0, key1);
+    const u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, tag}, keys);
+    T z[4];
+    Num<T>::box_muller(r.x, r.y, z[0], z[1]);
+    Num<T>::box_muller(r.z, r.w, z[2], z[3]);
+    out[i] = (double)z[j & 3];
+}
+
+}  // namespace mp
