@@ -98,6 +98,23 @@ def cpu_baseline(inp, seconds=12.0, steps=10000, chunk=16):
                       "1-core rate from %d particles" % (na, steps + 1, n_threads, n1)}
 
 
+def copy_calibration(nbytes):
+    """One device-to-device hipMemcpy of nbytes (a blit kernel on ROCm), the
+    known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +124,12 @@ def main():
     ap.add_argument("--particles", type=int, default=65536)
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run one device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build's path runs on a single MI355X "
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)

@@ -129,9 +129,24 @@ std::vector<T> cast_vec(const std::vector<double> &v)
     return std::vector<T>(v.begin(), v.end());
 }
 
+template <typename T, int kShape, bool kArray, bool kRenorm>
+void launch_heun(const mp::IndepArgs<T> &a, int grid, hipStream_t s, const float *htab)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (htab) {  // the table replaces per-lane field evaluation: one variant per noise kind
+            hipLaunchKernelGGL((mp::heun_indep_kernel<T, mp::kConstant, kArray, true, kRenorm>),
+                               dim3(grid), dim3(mp::kBlock), 0, s, a, htab);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((mp::heun_indep_kernel<T, kShape, kArray, false, kRenorm>), dim3(grid),
+                       dim3(mp::kBlock), 0, s, a, htab);
+}
+
 template <typename T, int kShape, bool kArray>
 void launch_step(const mp::IndepArgs<T> &a, int scheme, double eps, int max_iter,
-                 unsigned long long *it, unsigned long long *fl, int grid, hipStream_t s)
+                 unsigned long long *it, unsigned long long *fl, int grid, hipStream_t s,
+                 const float *htab)
 {
     if constexpr (sizeof(T) == 8) {
         if (scheme == MP_SCHEME_IMPLICIT_MIDPOINT) {
@@ -141,20 +156,22 @@ void launch_step(const mp::IndepArgs<T> &a, int scheme, double eps, int max_iter
             return;
         }
     }
-    hipLaunchKernelGGL((mp::heun_indep_kernel<T, kShape, kArray>), dim3(grid), dim3(mp::kBlock),
-                       0, s, a);
+    if (a.renorm)
+        launch_heun<T, kShape, kArray, true>(a, grid, s, htab);
+    else
+        launch_heun<T, kShape, kArray, false>(a, grid, s, htab);
 }
 
 template <typename T, bool kArray>
 void launch_shape(int shape_kind, const mp::IndepArgs<T> &a, int scheme, double eps,
                   int max_iter, unsigned long long *it, unsigned long long *fl, int grid,
-                  hipStream_t s)
+                  hipStream_t s, const float *htab)
 {
     switch (shape_kind) {
-    case mp::kSine: launch_step<T, mp::kSine, kArray>(a, scheme, eps, max_iter, it, fl, grid, s); break;
-    case mp::kSquare: launch_step<T, mp::kSquare, kArray>(a, scheme, eps, max_iter, it, fl, grid, s); break;
-    case mp::kConstant: launch_step<T, mp::kConstant, kArray>(a, scheme, eps, max_iter, it, fl, grid, s); break;
-    default: launch_step<T, mp::kMixed, kArray>(a, scheme, eps, max_iter, it, fl, grid, s); break;
+    case mp::kSine: launch_step<T, mp::kSine, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab); break;
+    case mp::kSquare: launch_step<T, mp::kSquare, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab); break;
+    case mp::kConstant: launch_step<T, mp::kConstant, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab); break;
+    default: launch_step<T, mp::kMixed, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab); break;
     }
 }
 
@@ -205,6 +222,10 @@ struct GroupRun : GroupBase {
     DevBuf<uint32_t> d_key0, d_key1, d_pid, d_status;
     DevBuf<int64_t> d_sched;
     DevBuf<unsigned long long> d_it, d_fl;
+    DevBuf<float> d_htab;      // uniform applied field of one launch (FP32 Heun)
+    bool uniform_field = false;
+    int field_shape0 = 0;
+    double happ0 = 0.0, fdt0 = 0.0;
     mp::IndepArgs<T> a{};
 
     GroupRun(const simulation::EnsembleSpec &sp, const Plan &plan, int gi, bool traj, bool ens)
@@ -253,6 +274,18 @@ struct GroupRun : GroupBase {
             pid[i] = (uint32_t)lp;
             shape_kind = shape_kind == -1 ? shape[i]
                                           : (shape_kind == shape[i] ? shape_kind : (int)mp::kMixed);
+        }
+        // one shared field table when every particle sees the same applied field
+        uniform_field = sizeof(T) == 4 && scheme == MP_SCHEME_HEUN && n > 0;
+        for (int i = 1; i < n && uniform_field; ++i)
+            uniform_field = shape[i] == shape[0] && happ[i] == happ[0] && fdt[i] == fdt[0];
+        if (uniform_field) {
+            field_shape0 = shape[0];
+            happ0 = happ[0];
+            fdt0 = fdt[0];
+            // groups of four aligned below n0, one entry past the last step, and
+            // the next group's prefetch: 4 * groups + 5 <= spl + 12
+            d_htab.alloc((size_t)spl + 16);
         }
         d_ex.upload(cast_vec<T>(ex));
         d_ey.upload(cast_vec<T>(ey));
@@ -325,10 +358,21 @@ struct GroupRun : GroupBase {
             a.n0 = n_done;
             a.n1 = n1;
             a.s_next = s_next;
+            const float *htab = nullptr;
+            if (uniform_field) {
+                const int64_t g0 = n_done >> 2, g1 = (n1 + 3) >> 2;
+                const int64_t count = 4 * (g1 - g0) + 5;
+                require((size_t)count <= d_htab.n, "field table too small");
+                hipLaunchKernelGGL(mp::field_table_kernel, dim3((unsigned)((count + 255) / 256)),
+                                   dim3(256), 0, s, field_shape0, happ0, fdt0, 4 * g0, count,
+                                   d_htab.p);
+                MP_HIP(hipGetLastError());
+                htab = d_htab.p;
+            }
             if (array_noise)
-                launch_shape<T, true>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s);
+                launch_shape<T, true>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s, htab);
             else
-                launch_shape<T, false>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s);
+                launch_shape<T, false>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s, htab);
             MP_HIP(hipGetLastError());
             ++launches;
             n_done = n1;

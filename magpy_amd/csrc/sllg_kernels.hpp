@@ -181,10 +181,69 @@ __device__ inline void record_sample(const IndepArgs<T> &a, int64_t s, int p, bo
 }
 
 // ------------------------------------------------------------------ Heun
+// One Heun step n -> n+1 of one particle (lib/integrators.cpp:372-405):
+// predictor m~ = m + L(m, h(m,t1) dt + s), corrector
+// m' = m + (L(m, v1) + L(m~, v2)) / 2 with s = sr sqrt(dt) w shared by both
+// stages; h1, h2 are the applied field at the two stage times.  The
+// anisotropy field enters as v = (m.ke) edt + s with ke = k e and
+// edt = e dt precomputed, which keeps each stage's dependency chain short.
+template <typename T, bool kRenorm>
+__device__ inline void heun_update(T &mx, T &my, T &mz, T sx, T sy, T sz, T h1, T h2, T kex,
+                                   T key, T kez, T edx, T edy, T edz, T dt, T al)
+{
+    const T sz1 = h1 * dt + sz, sz2 = h2 * dt + sz;
+    T d = mx * kex + my * key + mz * kez;
+    T vx = d * edx + sx, vy = d * edy + sy, vz = d * edz + sz1;
+    T ax, ay, az;
+    llg_map(mx, my, mz, vx, vy, vz, al, ax, ay, az);
+    const T tx = mx + ax, ty = my + ay, tz = mz + az;
+    d = tx * kex + ty * key + tz * kez;
+    vx = d * edx + sx;
+    vy = d * edy + sy;
+    vz = d * edz + sz2;
+    T bx, by, bz;
+    llg_map(tx, ty, tz, vx, vy, vz, al, bx, by, bz);
+    mx += T(0.5) * (ax + bx);
+    my += T(0.5) * (ay + by);
+    mz += T(0.5) * (az + bz);
+    if constexpr (kRenorm) {  // lib/simulation.cpp:379-387
+        const T nrm = sqrt(mx * mx + my * my + mz * mz);
+        mx = mx / nrm;
+        my = my / nrm;
+        mz = mz / nrm;
+    }
+}
+
+// Twelve Wiener normals of step group g (steps 4g..4g+3) scaled by sig.
+template <typename T>
+__device__ inline void group_normals(int64_t g, uint32_t pid, uint32_t tag, const PhiloxKeys &keys,
+                                     T sig, T z[12])
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const uint64_t c = (uint64_t)(3 * g + i);
+        const u32x4 w = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, tag}, keys);
+        Num<T>::box_muller(w.x, w.y, z[4 * i], z[4 * i + 1]);
+        Num<T>::box_muller(w.z, w.w, z[4 * i + 2], z[4 * i + 3]);
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i) z[i] *= sig;
+}
+
 // One block of steps [n0, n1) for independent particles.  Steps are walked in
-// groups of four, whose twelve normals come from three Philox calls.
-template <typename T, int kShape, bool kArrayNoise>
-__global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a)
+// groups of four, whose twelve normals come from three Philox calls.  The
+// state chain of one step depends on the previous step, so with one wave per
+// SIMD the kernel is latency bound; the normals of group g+1 are generated in
+// the same basic block as the four steps of group g, giving the scheduler an
+// independent instruction stream to fill the chain's latency.  Groups that
+// are wholly inside the block and record no sample take that fast path.  With
+// kTable the applied field of the whole launch is one shared table htab
+// (FP32 only): htab[i] = h(t_{4 g0 + 1 + i}) with g0 = n0 / 4 the first
+// group, read one group ahead; stage 2 of step n uses h(t_{n+2}), i.e.
+// t_{n+1} + dt up to one rounding of t.
+template <typename T, int kShape, bool kArrayNoise, bool kTable, bool kRenorm>
+__global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a,
+                                                            const float *__restrict__ htab)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < a.n;
@@ -193,85 +252,94 @@ __global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a)
     T mx = a.mx[q], my = a.my[q], mz = a.mz[q];
     const T ex = a.ex[q], ey = a.ey[q], ez = a.ez[q];
     const T k = a.k[q], al = a.alpha[q], dt = a.dt[q], sig = a.sig[q], amp = a.happ[q];
+    const T kex = k * ex, key = k * ey, kez = k * ez, edx = ex * dt, edy = ey * dt, edz = ez * dt;
     const double fred = a.fred[q], fdt = a.fdt[q], dtd = a.dtd[q];
     const int shape = a.shape[q];
     const double msw = a.msw[q];
     PhiloxKeys keys;
     keys.init(a.key0[q], a.key1[q]);
     const uint32_t pid = a.pid[q];
+    const int64_t n0 = a.n0, n1 = a.n1;
 
     int64_t s = a.s_next;
     int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
 
-    const int64_t g0 = a.n0 >> 2, g1 = (a.n1 + 3) >> 2;
-    for (int64_t g = g0; g < g1; ++g) {
-        T z[12];
-        if (!kArrayNoise) {
+    float hc[5], hn[5];  // table entries of the current and next group
+    auto fields = [&](int64_t n, int u, double base, T &h1, T &h2) {
+        if constexpr (kTable) {
+            h1 = (T)hc[u];
+            h2 = (T)hc[u + 1];
+        } else if constexpr (sizeof(T) == 4) {
+            const double c1 = (base + (double)(u + 1)) * fdt;
+            h1 = Applied<T, kShape>::at(shape, amp, c1);
+            h2 = Applied<T, kShape>::at(shape, amp, c1 + fdt);
+        } else {
+            const double t1 = (base + (double)(u + 1)) * dtd;
+            h1 = Applied<T, kShape>::at(shape, amp, fred, t1);
+            h2 = Applied<T, kShape>::at(shape, amp, fred, t1 + dtd);
+        }
+    };
+
+    T zc[12], zn[12];
+    const int64_t g_begin = n0 >> 2, g_end = (n1 + 3) >> 2;
+    if (!kArrayNoise) group_normals(g_begin, pid, a.tag, keys, sig, zc);
+    if constexpr (kTable) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const uint64_t c = (uint64_t)(3 * g + i);
-                const u32x4 w = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, a.tag}, keys);
-                Num<T>::box_muller(w.x, w.y, z[4 * i], z[4 * i + 1]);
-                Num<T>::box_muller(w.z, w.w, z[4 * i + 2], z[4 * i + 3]);
+        for (int j = 0; j < 5; ++j) hc[j] = htab[j];
+    }
+    for (int64_t g = g_begin; g < g_end; ++g) {
+        const double base = (double)(4 * g);
+        if (!kArrayNoise) group_normals(g + 1, pid, a.tag, keys, sig, zn);
+        if constexpr (kTable) {
+            const float *nxt = htab + 4 * (g + 1 - g_begin);
+#pragma unroll
+            for (int j = 0; j < 5; ++j) hn[j] = nxt[j];
+        }
+        const bool fast = 4 * g >= n0 && 4 * g + 4 <= n1 && next_rec > 4 * g + 4;
+        if (!kArrayNoise && fast) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                T h1, h2;
+                fields(4 * g + u, u, base, h1, h2);
+                heun_update<T, kRenorm>(mx, my, mz, zc[3 * u], zc[3 * u + 1], zc[3 * u + 2], h1,
+                                        h2, kex, key, kez, edx, edy, edz, dt, al);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t n = 4 * g + u;
+                if (n < n0 || n >= n1) continue;
+                T sx, sy, sz;
+                if (kArrayNoise) {
+                    const double *src = a.noise + ((size_t)n * (size_t)a.n + (size_t)q) * 3;
+                    sx = sig * (T)src[0];
+                    sy = sig * (T)src[1];
+                    sz = sig * (T)src[2];
+                } else {
+                    sx = zc[3 * u];
+                    sy = zc[3 * u + 1];
+                    sz = zc[3 * u + 2];
+                }
+                T h1, h2;
+                fields(n, u, base, h1, h2);
+                heun_update<T, kRenorm>(mx, my, mz, sx, sy, sz, h1, h2, kex, key, kez, edx, edy,
+                                        edz, dt, al);
+                if (n + 1 == next_rec) {
+                    do {
+                        record_sample(a, s, p, live, mx, my, mz, msw);
+                        ++s;
+                        next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+                    } while (next_rec == n + 1);
+                }
             }
         }
-        const double base = (double)(4 * g);
+        if (!kArrayNoise) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t n = 4 * g + u;
-            if (n < a.n0 || n >= a.n1) continue;
-            T w0, w1, w2;
-            if (kArrayNoise) {
-                const double *src = a.noise + ((size_t)n * (size_t)a.n + (size_t)q) * 3;
-                w0 = (T)src[0];
-                w1 = (T)src[1];
-                w2 = (T)src[2];
-            } else {
-                w0 = z[3 * u];
-                w1 = z[3 * u + 1];
-                w2 = z[3 * u + 2];
-            }
-            // field times: t1 = (n+1) dt, t2 = t1 + dt
-            T h1, h2;
-            if constexpr (sizeof(T) == 4) {
-                const double c1 = (base + (double)(u + 1)) * fdt;
-                h1 = Applied<T, kShape>::at(shape, amp, c1);
-                h2 = Applied<T, kShape>::at(shape, amp, c1 + fdt);
-            } else {
-                const double t1 = (base + (double)(u + 1)) * dtd;
-                h1 = Applied<T, kShape>::at(shape, amp, fred, t1);
-                h2 = Applied<T, kShape>::at(shape, amp, fred, t1 + dtd);
-            }
-            const T sx = sig * w0, sy = sig * w1, sz = sig * w2;
-            // stage 1: predictor
-            T d = (mx * ex + my * ey + mz * ez) * k;
-            T vx = (d * ex) * dt + sx, vy = (d * ey) * dt + sy, vz = (d * ez + h1) * dt + sz;
-            T ax, ay, az;
-            llg_map(mx, my, mz, vx, vy, vz, al, ax, ay, az);
-            const T tx = mx + ax, ty = my + ay, tz = mz + az;
-            // stage 2: corrector
-            d = (tx * ex + ty * ey + tz * ez) * k;
-            vx = (d * ex) * dt + sx;
-            vy = (d * ey) * dt + sy;
-            vz = (d * ez + h2) * dt + sz;
-            T bx, by, bz;
-            llg_map(tx, ty, tz, vx, vy, vz, al, bx, by, bz);
-            mx += T(0.5) * (ax + bx);
-            my += T(0.5) * (ay + by);
-            mz += T(0.5) * (az + bz);
-            if (a.renorm) {  // lib/simulation.cpp:379-387
-                const T nrm = sqrt(mx * mx + my * my + mz * mz);
-                mx = mx / nrm;
-                my = my / nrm;
-                mz = mz / nrm;
-            }
-            if (n + 1 == next_rec) {
-                do {
-                    record_sample(a, s, p, live, mx, my, mz, msw);
-                    ++s;
-                    next_rec = s < a.S ? a.sched[s] : INT64_MAX;
-                } while (next_rec == n + 1);
-            }
+            for (int i = 0; i < 12; ++i) zc[i] = zn[i];
+        }
+        if constexpr (kTable) {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) hc[j] = hn[j];
         }
     }
     if (live) {
@@ -280,6 +348,21 @@ __global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a)
         a.mz[p] = mz;
         if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[p] |= 1u;
     }
+}
+
+// Applied field of a uniform launch: out[i] = h(t_{base+1+i}), i < count,
+// evaluated in FP64 from the cycle count c = f_red t reduced to [0,1).
+__global__ void field_table_kernel(int shape, double amp, double fdt, int64_t base, int64_t count,
+                                   float *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const double c = (double)(base + 1 + i) * fdt;
+    const double fr = c - floor(c);
+    double h = amp;
+    if (shape == kSine) h = amp * sinpi(2.0 * fr);
+    else if (shape == kSquare) h = fr >= 0.5 ? -amp : amp;
+    out[i] = (float)h;
 }
 
 // ------------------------------------------------------------------ implicit midpoint
