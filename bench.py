@@ -34,6 +34,10 @@ MU0 = 1.25663706e-6
 METRIC = "particle-steps/s (sLLG Heun, 1 GPU) at SAR err ≤0.5% vs FP64; plain vs batched"
 FLOP_PER_PARTICLE_STEP = 278          # Heun, N = 1, no renorm
 PEAK_FP32_TFLOPS = 157.3              # MI355X vector FP32 (published)
+# HBM bytes of one default launch (65,536 particles x 131,072 steps), from
+# rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) scaled against a 1 GiB
+# device copy; see profiles/heun_indep_fp32_65536_ac_field.md.
+TRAFFIC_BYTES_DEFAULT_LAUNCH = 13.30e6
 
 
 def workload(n, seed=2024):
@@ -176,7 +180,10 @@ def main():
                    "run_length_steps": info1["n_steps"], "samples": max_samples,
                    "precision": args.precision, "state_finite": ok, "step_index": int(step)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": (TRAFFIC_BYTES_DEFAULT_LAUNCH
+                                 if (n, args.block, args.precision) == (65536, 131072, "fp32")
+                                 else None),
                      "kernel": "heun_indep_kernel<float, sine>",
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
                      "flop_per_particle_step": FLOP_PER_PARTICLE_STEP},
