@@ -192,12 +192,13 @@ struct HostCopy {
     std::vector<int32_t> shape;
 };
 
-// Results of one group copied back to the host.
+// Results of one group copied back to the host, per slot.
 struct GroupOut {
-    std::vector<double> state;   // [n][3] in group order
+    std::vector<int64_t> particle_of; // slot -> particle (-1 = padding)
+    std::vector<double> state;        // [slots][3]
     std::vector<uint32_t> status;
-    std::vector<double> samples; // [S][3][n] (empty when not recorded)
-    std::vector<double> ens;     // [S][3]   (empty when not recorded)
+    std::vector<double> samples;      // [S][3][slots] (empty when not recorded)
+    std::vector<double> ens;          // [S][3]        (empty when not recorded)
     int64_t newton_iters = 0, newton_fail = 0;
 };
 
@@ -205,20 +206,70 @@ struct GroupBase {
     int64_t n_done = 0, n_total = 0, launches = 0;
     virtual ~GroupBase() = default;
     virtual void enqueue(int64_t upto, hipStream_t s) = 0;
-    virtual void state(std::vector<double> &out) = 0;  // [n][3] group order
+    virtual void state(std::vector<double> &out) = 0;  // [slots][3]
+    virtual const std::vector<int64_t> &slots() const = 0;
     virtual GroupOut collect(hipStream_t s) = 0;
 };
 
+// Cluster packing: members of a group sorted by size (largest first), each
+// placed in the first block with room; a member never straddles blocks.
+// Returns slot -> particle (-1 padding), and per slot the block-relative
+// start lane of its member and the member size.
+void pack_clusters(const std::vector<std::pair<int64_t, int64_t>> &members, int block,
+                   std::vector<int64_t> &slot_particle, std::vector<int> &cbase,
+                   std::vector<int> &csize)
+{
+    std::vector<size_t> order(members.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return members[x].second - members[x].first > members[y].second - members[y].first;
+    });
+    std::vector<int> fill;                       // lanes used per block
+    std::vector<std::vector<size_t>> content;    // members per block
+    size_t first_open = 0;
+    for (size_t oi : order) {
+        const int sz = (int)(members[oi].second - members[oi].first);
+        size_t bi = first_open;
+        while (bi < fill.size() && fill[bi] + sz > block) ++bi;
+        if (bi == fill.size()) {
+            fill.push_back(0);
+            content.emplace_back();
+        }
+        fill[bi] += sz;
+        content[bi].push_back(oi);
+        while (first_open < fill.size() && fill[first_open] == block) ++first_open;
+    }
+    const size_t nslots = fill.size() * (size_t)block;
+    slot_particle.assign(nslots, -1);
+    cbase.assign(nslots, 0);
+    csize.assign(nslots, 0);
+    for (size_t bi = 0; bi < content.size(); ++bi) {
+        int lane = 0;
+        for (size_t oi : content[bi]) {
+            const int sz = (int)(members[oi].second - members[oi].first);
+            for (int j = 0; j < sz; ++j) {
+                const size_t slot = bi * (size_t)block + (size_t)(lane + j);
+                slot_particle[slot] = members[oi].first + j;
+                cbase[slot] = lane;
+                csize[slot] = sz;
+            }
+            lane += sz;
+        }
+    }
+}
+
 template <typename T>
 struct GroupRun : GroupBase {
-    int n = 0, grid = 0, n_waves = 0, scheme = 0, shape_kind = -1, max_iter = 1000;
+    int n = 0, grid = 0, block = mp::kBlock, n_waves = 0, scheme = 0, shape_kind = -1;
+    int max_iter = 1000;
     double eps = 1e-9;
-    bool array_noise = false, want_traj = false, want_ens = false;
+    bool array_noise = false, want_traj = false, want_ens = false, cluster = false;
     int64_t S = 0, s_next = 0, spl = 1;
-    std::vector<int64_t> sched;
+    std::vector<int64_t> sched, slot_particle;
     DevBuf<T> d_ex, d_ey, d_ez, d_k, d_al, d_dt, d_sig, d_sr, d_happ, d_mx, d_my, d_mz, d_samples;
+    DevBuf<T> d_px, d_py, d_pz, d_w;
     DevBuf<double> d_fred, d_fdt, d_dtd, d_msw, d_noise, d_partial, d_ens;
-    DevBuf<int> d_shape;
+    DevBuf<int> d_shape, d_cbase, d_csize;
     DevBuf<uint32_t> d_key0, d_key1, d_pid, d_status;
     DevBuf<int64_t> d_sched;
     DevBuf<unsigned long long> d_it, d_fl;
@@ -228,11 +279,13 @@ struct GroupRun : GroupBase {
     double happ0 = 0.0, fdt0 = 0.0;
     mp::IndepArgs<T> a{};
 
-    GroupRun(const simulation::EnsembleSpec &sp, const Plan &plan, int gi, bool traj, bool ens)
+    // cluster_block == 0: one slot per particle of the group (independent
+    // particles); otherwise members packed into blocks of cluster_block lanes.
+    GroupRun(const simulation::EnsembleSpec &sp, const Plan &plan, int gi, bool traj, bool ens,
+             int cluster_block)
     {
         const int64_t b = plan.group_begin[(size_t)gi], e = plan.group_begin[(size_t)gi + 1];
         const int64_t P = sp.member_offsets[sp.n_members];
-        n = (int)(e - b);
         sched = plan.sched[(size_t)gi];
         S = (int64_t)sched.size();
         n_total = plan.group_steps[(size_t)gi];
@@ -243,13 +296,32 @@ struct GroupRun : GroupBase {
         want_traj = traj;
         want_ens = ens;
         spl = steps_per_launch(sp.scheme, sp.precision);
+        cluster = cluster_block > 0;
+        std::vector<int> cbase, csize;
+        if (cluster) {
+            std::vector<std::pair<int64_t, int64_t>> members;  // particle ranges
+            for (int64_t i = b; i < e;) {
+                const int64_t m = plan.member_of[(size_t)plan.perm[(size_t)i]];
+                const int64_t len = sp.member_offsets[m + 1] - sp.member_offsets[m];
+                members.emplace_back(sp.member_offsets[m], sp.member_offsets[m] + len);
+                i += len;
+            }
+            block = cluster_block;
+            pack_clusters(members, block, slot_particle, cbase, csize);
+        } else {
+            slot_particle.assign(plan.perm.begin() + b, plan.perm.begin() + e);
+        }
+        n = (int)slot_particle.size();
 
-        std::vector<double> ex(n), ey(n), ez(n), k(n), al(n), dt(n), sig(n), sr(n), happ(n),
-            fred(n), fdt(n), msw(n), mx(n), my(n), mz(n);
-        std::vector<int> shape(n);
-        std::vector<uint32_t> key0(n), key1(n), pid(n);
+        std::vector<double> ex(n, 0.0), ey(n, 0.0), ez(n, 1.0), k(n, 0.0), al(n, 0.0), dt(n, 0.0),
+            sig(n, 0.0), sr(n, 0.0), happ(n, 0.0), fred(n, 0.0), fdt(n, 0.0), msw(n, 0.0),
+            mx(n, 0.0), my(n, 0.0), mz(n, 1.0), px(n, 0.0), py(n, 0.0), pz(n, 0.0), w(n, 0.0);
+        std::vector<int> shape(n, MP_FIELD_CONSTANT);
+        std::vector<uint32_t> key0(n, 0), key1(n, 0), pid(n, 0);
+        bool first = true;
         for (int i = 0; i < n; ++i) {
-            const int64_t p = plan.perm[(size_t)(b + i)];
+            const int64_t p = slot_particle[(size_t)i];
+            if (p < 0) continue;
             const int64_t m = plan.member_of[(size_t)p];
             const simulation::ReducedMember &r = plan.red[(size_t)m];
             const int64_t lp = p - sp.member_offsets[m];
@@ -272,17 +344,32 @@ struct GroupRun : GroupBase {
             key0[i] = (uint32_t)(uint64_t)sp.seed[m];
             key1[i] = (uint32_t)((uint64_t)sp.seed[m] >> 32);
             pid[i] = (uint32_t)lp;
-            shape_kind = shape_kind == -1 ? shape[i]
-                                          : (shape_kind == shape[i] ? shape_kind : (int)mp::kMixed);
+            if (cluster) {
+                // lib/simulation.cpp:577-583 and lib/field.cpp:212-215
+                const double vroot = std::pow(r.average_volume, 1. / 3);
+                px[i] = sp.location[3 * p] / vroot;
+                py[i] = sp.location[3 * p + 1] / vroot;
+                pz[i] = sp.location[3 * p + 2] / vroot;
+                const double pre = kMU0 * sp.magnetisation[m] * sp.magnetisation[m] / 8.0 /
+                                   kPiHost / r.average_anisotropy;
+                w[i] = pre * r.v_red[(size_t)lp];
+            }
+            if (first) {
+                shape_kind = shape[i];
+                field_shape0 = shape[i];
+                happ0 = happ[i];
+                fdt0 = fdt[i];
+                first = false;
+            } else if (shape_kind != shape[i]) {
+                shape_kind = (int)mp::kMixed;
+            }
         }
         // one shared field table when every particle sees the same applied field
-        uniform_field = sizeof(T) == 4 && scheme == MP_SCHEME_HEUN && n > 0;
-        for (int i = 1; i < n && uniform_field; ++i)
-            uniform_field = shape[i] == shape[0] && happ[i] == happ[0] && fdt[i] == fdt[0];
+        uniform_field = sizeof(T) == 4 && scheme == MP_SCHEME_HEUN && !first;
+        for (int i = 0; i < n && uniform_field; ++i)
+            if (slot_particle[(size_t)i] >= 0)
+                uniform_field = shape[i] == field_shape0 && happ[i] == happ0 && fdt[i] == fdt0;
         if (uniform_field) {
-            field_shape0 = shape[0];
-            happ0 = happ[0];
-            fdt0 = fdt[0];
             // groups of four aligned below n0, one entry past the last step, and
             // the next group's prefetch: 4 * groups + 5 <= spl + 12
             d_htab.alloc((size_t)spl + 16);
@@ -311,8 +398,16 @@ struct GroupRun : GroupBase {
         d_sched.upload(sched);
         d_it.upload(std::vector<unsigned long long>((size_t)n, 0));
         d_fl.upload(std::vector<unsigned long long>((size_t)n, 0));
-        grid = (n + mp::kBlock - 1) / mp::kBlock;
-        n_waves = grid * (mp::kBlock / mp::kWave);
+        if (cluster) {
+            d_px.upload(cast_vec<T>(px));
+            d_py.upload(cast_vec<T>(py));
+            d_pz.upload(cast_vec<T>(pz));
+            d_w.upload(cast_vec<T>(w));
+            d_cbase.upload(cbase);
+            d_csize.upload(csize);
+        }
+        grid = (n + block - 1) / block;
+        n_waves = (grid * block) / mp::kWave;
         if (want_traj) d_samples.alloc((size_t)S * 3 * (size_t)n);
         if (want_ens) {
             // each recorded sample writes all of its (component, wave) slots;
@@ -321,10 +416,11 @@ struct GroupRun : GroupBase {
             d_ens.alloc((size_t)S * 3);
         }
         if (array_noise) {
-            std::vector<double> nz((size_t)n_total * (size_t)n * 3);
+            std::vector<double> nz((size_t)n_total * (size_t)n * 3, 0.0);
             for (int64_t st = 0; st < n_total; ++st)
                 for (int i = 0; i < n; ++i) {
-                    const int64_t p = plan.perm[(size_t)(b + i)];
+                    const int64_t p = slot_particle[(size_t)i];
+                    if (p < 0) continue;
                     for (int d = 0; d < 3; ++d)
                         nz[((size_t)st * n + i) * 3 + d] = sp.noise_array[((size_t)st * P + p) * 3 + d];
                 }
@@ -349,6 +445,41 @@ struct GroupRun : GroupBase {
         a.noise = array_noise ? d_noise.p : nullptr;
     }
 
+    const std::vector<int64_t> &slots() const override { return slot_particle; }
+
+    void launch_cluster(hipStream_t s, const float *htab)
+    {
+        mp::ClusterArgs<T> ca{a, d_px.p, d_py.p, d_pz.p, d_w.p, d_cbase.p, d_csize.p};
+        const size_t shmem = 3 * (size_t)block * sizeof(mp::Vec4<T>);
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, s, ca, htab);
+        };
+        const bool rn = a.renorm != 0;
+        if (htab) {
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false>);
+            return;
+        }
+        switch (shape_kind) {
+        case mp::kSine:
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true>);
+            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false>);
+            break;
+        case mp::kSquare:
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true>);
+            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false>);
+            break;
+        case mp::kConstant:
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false>);
+            break;
+        default:
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true>);
+            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false>);
+            break;
+        }
+    }
+
     void enqueue(int64_t upto, hipStream_t s) override
     {
         upto = std::min(upto, n_total);
@@ -369,7 +500,9 @@ struct GroupRun : GroupBase {
                 MP_HIP(hipGetLastError());
                 htab = d_htab.p;
             }
-            if (array_noise)
+            if (cluster)
+                launch_cluster(s, htab);
+            else if (array_noise)
                 launch_shape<T, true>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s, htab);
             else
                 launch_shape<T, false>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s, htab);
@@ -396,6 +529,7 @@ struct GroupRun : GroupBase {
     GroupOut collect(hipStream_t s) override
     {
         GroupOut o;
+        o.particle_of = slot_particle;
         if (want_ens) {
             hipLaunchKernelGGL(mp::ens_reduce_kernel, dim3((unsigned)(S * 3)), dim3(mp::kBlock), 0,
                                s, d_partial.p, n_waves, S * 3, d_ens.p);
@@ -537,16 +671,36 @@ void validate(const EnsembleSpec &sp)
     }
     if (sp.interactions) {
         for (int64_t m = 0; m < sp.n_members; ++m)
-            if (sp.member_offsets[m + 1] - sp.member_offsets[m] > 1)
-                throw Error(MP_ERR_UNSUPPORTED,
-                            "dipolar interactions between particles of one cluster are not "
-                            "built into this library yet; pass interactions=0");
+            if (sp.member_offsets[m + 1] - sp.member_offsets[m] > 1) {
+                if (sp.scheme == MP_SCHEME_IMPLICIT_MIDPOINT)
+                    throw Error(MP_ERR_UNSUPPORTED,
+                                "the implicit midpoint scheme with dipolar interactions is not "
+                                "built into this library yet; use the Heun scheme or "
+                                "interactions=0");
+                if (sp.noise == MP_NOISE_ARRAY)
+                    throw Error(MP_ERR_UNSUPPORTED,
+                                "caller-given Wiener increments are not supported for "
+                                "interacting clusters; use the Philox stream");
+            }
     }
 }
 
 }  // namespace simulation
 
 namespace {
+
+// Block size for interacting clusters: the smallest multiple of the wave
+// (>= 256 lanes, for barrier amortisation) that holds the largest cluster.
+int cluster_block_size(int64_t nmax)
+{
+    if (const char *env = std::getenv("MAGPY_AMD_CLUSTER_BLOCK")) {
+        const int v = std::atoi(env);
+        if (v >= nmax && v % mp::kWave == 0 && v <= 1024) return v;
+    }
+    for (int b = 256; b <= 1024; b *= 2)
+        if (b >= nmax) return b;
+    return 0;
+}
 
 mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool ens)
 {
@@ -610,11 +764,25 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
         for (int g = 0; g < G; ++g)
             require(plan.group_steps[(size_t)g] <= sp.noise_steps,
                     "Exceeded allocated random number array size (noise_steps < steps)");
+    // interacting clusters: one block holds whole clusters, so the block must
+    // be at least the largest cluster
+    int cluster_block = 0;
+    if (sp.interactions) {
+        int64_t nmax = 1;
+        for (int64_t m = 0; m < sp.n_members; ++m)
+            nmax = std::max(nmax, sp.member_offsets[m + 1] - sp.member_offsets[m]);
+        if (nmax > 1) {
+            cluster_block = cluster_block_size(nmax);
+            if (cluster_block == 0)
+                throw simulation::Error(MP_ERR_UNSUPPORTED,
+                                        "clusters of more than 1024 particles are not supported");
+        }
+    }
     for (int g = 0; g < G; ++g) {
         if (sp.precision == MP_PRECISION_FP32 && sp.scheme == MP_SCHEME_HEUN)
-            ss->groups.emplace_back(new GroupRun<float>(sp, plan, g, traj, ens));
+            ss->groups.emplace_back(new GroupRun<float>(sp, plan, g, traj, ens, cluster_block));
         else
-            ss->groups.emplace_back(new GroupRun<double>(sp, plan, g, traj, ens));
+            ss->groups.emplace_back(new GroupRun<double>(sp, plan, g, traj, ens, cluster_block));
     }
     return ss.release();
 }
@@ -648,11 +816,11 @@ void session_finish(mp_session *ss, mp_ensemble_out &out)
     int64_t newton_iters = 0, newton_fail = 0;
     for (size_t gi = 0; gi < ss->groups.size(); ++gi) {
         GroupOut o = ss->groups[gi]->collect(ss->sg.s);
-        const int64_t b = plan.group_begin[gi], e = plan.group_begin[gi + 1];
-        const int n = (int)(e - b);
+        const int n = (int)o.particle_of.size();
         const std::vector<int64_t> &sched = plan.sched[gi];
         for (int i = 0; i < n; ++i) {
-            const int64_t p = plan.perm[(size_t)(b + i)];
+            const int64_t p = o.particle_of[(size_t)i];
+            if (p < 0) continue;
             for (int d = 0; d < 3; ++d) state[(size_t)p * 3 + d] = o.state[(size_t)i * 3 + d];
             status[(size_t)p] = o.status[(size_t)i];
         }
@@ -662,7 +830,8 @@ void session_finish(mp_session *ss, mp_ensemble_out &out)
         if (want_traj)
             for (int64_t s = 0; s < S; ++s)
                 for (int i = 0; i < n; ++i) {
-                    const int64_t p = plan.perm[(size_t)(b + i)];
+                    const int64_t p = o.particle_of[(size_t)i];
+                    if (p < 0) continue;
                     for (int d = 0; d < 3; ++d)
                         traj[((size_t)s * 3 + d) * (size_t)P + (size_t)p] =
                             sched[(size_t)s] == 0
@@ -676,7 +845,8 @@ void session_finish(mp_session *ss, mp_ensemble_out &out)
                     if (sched[(size_t)s] == 0) {
                         v = 0.0;
                         for (int i = 0; i < n; ++i) {
-                            const int64_t p = plan.perm[(size_t)(b + i)];
+                            const int64_t p = o.particle_of[(size_t)i];
+                            if (p < 0) continue;
                             v += h.m0[(size_t)p * 3 + d] * h.ms[(size_t)plan.member_of[(size_t)p]];
                         }
                     }
@@ -853,14 +1023,14 @@ int mp_session_state(mp_session *session, double *state, int64_t *step)
         if (!session || !state) throw simulation::Error(MP_ERR_ARG, "session and state are required");
         MP_HIP(hipSetDevice(session->device));
         MP_HIP(hipStreamSynchronize(session->sg.s));
-        const Plan &plan = session->plan;
         for (size_t gi = 0; gi < session->groups.size(); ++gi) {
             std::vector<double> st;
             session->groups[gi]->state(st);
-            const int64_t b = plan.group_begin[gi], e = plan.group_begin[gi + 1];
-            for (int64_t i = 0; i < e - b; ++i) {
-                const int64_t p = plan.perm[(size_t)(b + i)];
-                for (int d = 0; d < 3; ++d) state[p * 3 + d] = st[(size_t)i * 3 + d];
+            const std::vector<int64_t> &slots = session->groups[gi]->slots();
+            for (size_t i = 0; i < slots.size(); ++i) {
+                const int64_t p = slots[i];
+                if (p < 0) continue;
+                for (int d = 0; d < 3; ++d) state[p * 3 + d] = st[i * 3 + d];
             }
         }
         if (step) *step = session->groups[0]->n_done;

@@ -365,6 +365,164 @@ __global__ void field_table_kernel(int shape, double amp, double fdt, int64_t ba
     out[i] = (float)h;
 }
 
+// ------------------------------------------------------------------ interacting clusters
+// Clusters of several particles with dipolar coupling (lib/field.cpp:187-225):
+//   h_i += sum_{j != i} w_j / r_ij^3 (3 (m_j.u_ij) u_ij - m_j),  w_j = P v_j,
+//   P = mu0 Ms^2 / (8 pi K_avg), u_ij = (p_j - p_i) / |p_j - p_i|,
+// with positions already divided by V_avg^(1/3).  Clusters are packed
+// first-fit-decreasing into thread blocks, one lane per particle, a cluster
+// never straddling two blocks; every lane of a block (padding included) runs
+// the same step loop, so block barriers are uniform.  The stage states are
+// exchanged through LDS, double-buffered so one barrier per stage suffices.
+template <typename T>
+struct ClusterArgs {
+    IndepArgs<T> base;      // per-slot arrays (n = slots, padding included)
+    const T *px, *py, *pz;  // reduced positions
+    const T *w;             // P * v_red of the particle
+    const int *cbase;       // block-relative lane of the slot's cluster start
+    const int *csize;       // cluster size (0 for padding)
+};
+
+template <typename T>
+struct alignas(16) Vec4 {
+    T x, y, z, w;
+};
+
+// Dipolar field at lane `self` from lanes [b, e) of its cluster except itself.
+template <typename T>
+__device__ inline void dipolar_sum(const Vec4<T> *pos, const Vec4<T> *m, int self, int b, int e,
+                                   T &hx, T &hy, T &hz)
+{
+    const Vec4<T> pi = pos[self];
+    hx = hy = hz = T(0);
+    auto term = [&](int j) {
+        const Vec4<T> pj = pos[j];
+        const Vec4<T> mj = m[j];
+        const T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        T rinv;
+        if constexpr (sizeof(T) == 4) rinv = __builtin_amdgcn_rsqf(r2);
+        else rinv = 1.0 / sqrt(r2);
+        const T rinv2 = rinv * rinv;
+        const T c = pj.w * rinv2 * rinv;
+        const T t = T(3) * (mj.x * dx + mj.y * dy + mj.z * dz) * rinv2;
+        hx += c * (t * dx - mj.x);
+        hy += c * (t * dy - mj.y);
+        hz += c * (t * dz - mj.z);
+    };
+    for (int j = b; j < self; ++j) term(j);
+    for (int j = self + 1; j < e; ++j) term(j);
+}
+
+template <typename T, int kShape, bool kTable, bool kRenorm>
+__global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
+                                                           const float *__restrict__ htab)
+{
+    const IndepArgs<T> &a = ca.base;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    Vec4<T> *pos = reinterpret_cast<Vec4<T> *>(smem_raw);
+    Vec4<T> *mb0 = pos + blockDim.x;
+    Vec4<T> *mb1 = mb0 + blockDim.x;
+
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x * blockDim.x + tid;   // slot: the grid covers all slots exactly
+    const int cn = ca.csize[p];
+    const bool live = cn > 0;
+    const int cb = ca.cbase[p], ce = cb + cn;
+
+    T mx = a.mx[p], my = a.my[p], mz = a.mz[p];
+    const T ex = a.ex[p], ey = a.ey[p], ez = a.ez[p];
+    const T k = a.k[p], al = a.alpha[p], dt = a.dt[p], sig = a.sig[p], amp = a.happ[p];
+    const T kex = k * ex, key = k * ey, kez = k * ez, edx = ex * dt, edy = ey * dt, edz = ez * dt;
+    const double fred = a.fred[p], fdt = a.fdt[p], dtd = a.dtd[p];
+    const int shape = a.shape[p];
+    const double msw = a.msw[p];
+    PhiloxKeys keys;
+    keys.init(a.key0[p], a.key1[p]);
+    const uint32_t pid = a.pid[p];
+    const int64_t n0 = a.n0, n1 = a.n1;
+    pos[tid] = Vec4<T>{ca.px[p], ca.py[p], ca.pz[p], ca.w[p]};
+
+    int64_t s = a.s_next;
+    int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+    T z[4];
+    uint64_t have = UINT64_MAX;
+
+    for (int64_t n = n0; n < n1; ++n) {
+        T w3[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const uint64_t j = 3 * (uint64_t)n + d;
+            const uint64_t c = j >> 2;
+            if (c != have) {
+                const u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, a.tag}, keys);
+                Num<T>::box_muller(r.x, r.y, z[0], z[1]);
+                Num<T>::box_muller(r.z, r.w, z[2], z[3]);
+                have = c;
+            }
+            const int jj = (int)(j & 3);
+            w3[d] = jj == 0 ? z[0] : jj == 1 ? z[1] : jj == 2 ? z[2] : z[3];
+        }
+        const T sx = sig * w3[0], sy = sig * w3[1], sz = sig * w3[2];
+        T h1, h2;
+        if constexpr (kTable) {
+            h1 = (T)htab[n - 4 * (n0 >> 2)];
+            h2 = (T)htab[n - 4 * (n0 >> 2) + 1];
+        } else if constexpr (sizeof(T) == 4) {
+            const double c1 = (double)(n + 1) * fdt;
+            h1 = Applied<T, kShape>::at(shape, amp, c1);
+            h2 = Applied<T, kShape>::at(shape, amp, c1 + fdt);
+        } else {
+            const double t1 = (double)(n + 1) * dtd;
+            h1 = Applied<T, kShape>::at(shape, amp, fred, t1);
+            h2 = Applied<T, kShape>::at(shape, amp, fred, t1 + dtd);
+        }
+        // stage 1
+        mb0[tid] = Vec4<T>{mx, my, mz, T(0)};
+        __syncthreads();
+        T hx, hy, hz;
+        dipolar_sum(pos, mb0, tid, cb, ce, hx, hy, hz);
+        T d = mx * kex + my * key + mz * kez;
+        T vx = d * edx + (hx * dt + sx), vy = d * edy + (hy * dt + sy),
+          vz = d * edz + ((hz + h1) * dt + sz);
+        T ax, ay, az;
+        llg_map(mx, my, mz, vx, vy, vz, al, ax, ay, az);
+        const T tx = mx + ax, ty = my + ay, tz = mz + az;
+        // stage 2
+        mb1[tid] = Vec4<T>{tx, ty, tz, T(0)};
+        __syncthreads();
+        dipolar_sum(pos, mb1, tid, cb, ce, hx, hy, hz);
+        d = tx * kex + ty * key + tz * kez;
+        vx = d * edx + (hx * dt + sx);
+        vy = d * edy + (hy * dt + sy);
+        vz = d * edz + ((hz + h2) * dt + sz);
+        T bx, by, bz;
+        llg_map(tx, ty, tz, vx, vy, vz, al, bx, by, bz);
+        mx += T(0.5) * (ax + bx);
+        my += T(0.5) * (ay + by);
+        mz += T(0.5) * (az + bz);
+        if constexpr (kRenorm) {
+            if (live) {
+                const T nrm = sqrt(mx * mx + my * my + mz * mz);
+                mx = mx / nrm;
+                my = my / nrm;
+                mz = mz / nrm;
+            }
+        }
+        if (n + 1 == next_rec) {
+            do {
+                record_sample(a, s, p, live, mx, my, mz, msw);
+                ++s;
+                next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+            } while (next_rec == n + 1);
+        }
+    }
+    a.mx[p] = mx;
+    a.my[p] = my;
+    a.mz[p] = mz;
+    if (live && !(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[p] |= 1u;
+}
+
 // ------------------------------------------------------------------ implicit midpoint
 // FP64, one lane per particle.  Restates integrator::implicit_midpoint
 // (lib/integrators.cpp:576-651) with optimisation::newton_raphson_noinv
