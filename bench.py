@@ -1,22 +1,31 @@
-"""Throughput of the sLLG Heun ensemble path on one MI355X.
+"""Throughput of the sLLG ensemble path on one MI355X.
 
-Workload (the 65,536-particle AC-field ensemble): 65,536 independent
-particles, radii lognormal (median 10 nm, sigma 0.1), K = 23 kJ/m^3, easy
-axes uniform on the sphere, m0 along the easy axis, alpha = 0.1, T = 300 K,
-sinusoidal z field 20 mT at 300 kHz, Heun, FP32, dt = 1e-14 s, run length
-5 field cycles (1.67e9 steps) with 2,000 samples per cycle reduced on the
-device.  Inputs are synthetic and resident in HBM before timing starts.
+Default workload (the headline line): the 65,536-particle AC-field ensemble —
+65,536 independent particles, radii lognormal (median 10 nm, sigma 0.1),
+K = 23 kJ/m^3, easy axes uniform on the sphere, m0 along the easy axis,
+alpha = 0.1, T = 300 K, sinusoidal z field 20 mT at 300 kHz, Heun, FP32,
+dt = 1e-14 s, run length 5 field cycles (1.67e9 steps) with 2,000 samples per
+cycle reduced on the device.  Inputs are synthetic and resident in HBM
+before timing starts.
 
-One bench "step" is one step block: BLOCK time steps of all particles, one
-kernel launch.  value = particles * BLOCK * steps / wall time of the timed
-blocks (particle-steps/s).  The roofline object divides the algorithmic FP32
-work of one launch (278 FLOP per particle-step, counted from the reference's
-formulas) by the average launch duration measured with HIP events on the
-launch stream.  The CPU baseline times the FP64 oracle (a reference-
-structured restatement of simulation::full_dynamics with the reference's
-mt19937_64 Wiener source) on a bounded sample of the same workload.
+Other workloads (--workload):
+  clusters  4,096 interacting clusters, N uniform in [2, 64], 10% volume
+            fraction, centres >= 2.2 radii apart, radius 10 nm, zero field,
+            Heun FP32, dt = 1e-14 s, 2e5 steps.
+  implicit  the 65,536-particle AC-field ensemble with the implicit midpoint
+            scheme at dt = 5e-13 s (FP64, per-particle Newton); the run ends
+            after the timed blocks so the Newton counts can be reported.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--block B]
+One bench "step" is one step block: BLOCK time steps of every particle, one
+kernel launch.  value = particle-steps / wall time of the timed blocks.  The
+roofline object divides the algorithmic work of one launch (FLOP counted
+from the reference's formulas) by the average launch duration measured with
+HIP events on the launch stream.  The CPU baseline times the FP64 oracle (a
+reference-structured restatement of simulation::full_dynamics with the
+reference's mt19937_64 Wiener source) on the host cores on a bounded sample
+of the same workload.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--block B] [--workload W]
 """
 import argparse
 import json
@@ -33,14 +42,16 @@ sys.path.insert(0, ROOT)
 MU0 = 1.25663706e-6
 METRIC = "particle-steps/s (sLLG Heun, 1 GPU) at SAR err ≤0.5% vs FP64; plain vs batched"
 FLOP_PER_PARTICLE_STEP = 278          # Heun, N = 1, no renorm
-PEAK_FP32_TFLOPS = 157.3              # MI355X vector FP32 (published)
+FLOP_PER_PAIR_STEP = 48               # 24 FLOP per ordered pair, two field evaluations
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (published)
 # HBM bytes of one default launch (65,536 particles x 131,072 steps), from
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) scaled against a 1 GiB
 # device copy; see profiles/heun_indep_fp32_65536_ac_field.md.
 TRAFFIC_BYTES_DEFAULT_LAUNCH = 13.30e6
+DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256)}
 
 
-def workload(n, seed=2024):
+def ac_field_inputs(n, seed=2024):
     rng = np.random.default_rng(seed)
     r = 10e-9 * np.exp(0.1 * rng.normal(size=n))
     th = 2 * np.pi * rng.uniform(size=n)
@@ -54,32 +65,81 @@ def workload(n, seed=2024):
                 seeds=rng.integers(0, 2 ** 31 - 1, size=n))
 
 
-def cpu_baseline(inp, seconds=12.0, steps=10000, chunk=16):
+def cluster_inputs(n_clusters, seed=2025):
+    from magpy_amd import geometry as G
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(2, 65, size=n_clusters)
+    e = G.cluster_ensemble(sizes, 10e-9, 0.10, 2.2, rng)
+    e.update(magnetisation=400e3, damping=0.1, temperature=300.0, field_shape='constant',
+             field_amplitude=0.0, field_frequency=0.0,
+             seeds=rng.integers(0, 2 ** 31 - 1, size=n_clusters))
+    return e
+
+
+def workload(name, n, timed_steps):
+    """(inputs, session kwargs, algorithmic FLOP per time step, precision, description)."""
+    if name == "ac_field":
+        inp = ac_field_inputs(n)
+        kw = dict(time_step=1e-14, end_time=5 / 3e5, max_samples=2000 * 5 + 1,
+                  interactions=False, use_implicit=False)
+        return (inp, kw, float(FLOP_PER_PARTICLE_STEP * n), "fp32",
+                "65,536-particle AC-field Heun ensemble (20 mT, 300 kHz, dt=1e-14 s)")
+    if name == "implicit":
+        inp = ac_field_inputs(n)
+        kw = dict(time_step=5e-13, end_time=timed_steps * 5e-13, max_samples=2,
+                  interactions=False, use_implicit=True)
+        return (inp, kw, None, "fp64",
+                "65,536-particle AC-field ensemble, implicit midpoint FP64 (dt=5e-13 s)")
+    if name == "clusters":
+        inp = cluster_inputs(n)
+        sizes = np.diff(inp["member_offsets"])
+        flop = float(FLOP_PER_PARTICLE_STEP * sizes.sum()
+                     + FLOP_PER_PAIR_STEP * np.sum(sizes * (sizes - 1)))
+        kw = dict(time_step=1e-14, end_time=2e5 * 1e-14, max_samples=201, interactions=True,
+                  use_implicit=False)
+        return (inp, kw, flop, "fp32",
+                "4,096 interacting clusters N~U{2..64}, 10% volume fraction, Heun FP32 "
+                "(dt=1e-14 s, 2e5 steps)")
+    raise SystemExit("unknown workload " + name)
+
+
+def cpu_baseline(name, inp, seconds=12.0):
     """Time the oracle on the host cores (threads release the GIL in ctypes)."""
     from oracle import oracle as O
     O.lib()
     n_threads = min(os.cpu_count() or 1, 16)
-    dt = 1e-14
-    end = steps * dt
+    offs = inp["member_offsets"]
+    n_members = len(offs) - 1
+    if name == "ac_field":
+        dt, steps, implicit, interactions = 1e-14, 10000, False, False
+    elif name == "implicit":
+        dt, steps, implicit, interactions = 5e-13, 100, True, False
+    else:
+        dt, steps, implicit, interactions = 1e-14, 200, False, True
     lock = threading.Lock()
     state = {"next": 0, "done": 0}
     t_stop = [0.0]
+
+    def member(i):
+        b, e = int(offs[i]), int(offs[i + 1])
+        o = O.full_dynamics(inp["radius"][b:e], inp["anisotropy"][b:e], inp["anisotropy_axis"][b:e],
+                            inp["magnetisation_direction"][b:e], inp["location"][b:e], 400e3, 0.1,
+                            300.0, False, interactions, implicit, 1e-9, dt, steps * dt, 2,
+                            int(inp["seeds"][i]), field_shape=inp["field_shape"],
+                            field_amplitude=inp["field_amplitude"],
+                            field_frequency=inp["field_frequency"], rng="mt")
+        return (e - b) * o["stats"].n_steps
 
     def worker():
         while True:
             with lock:
                 if time.perf_counter() > t_stop[0]:
                     return
-                i0 = state["next"] % len(inp["radius"])
-                state["next"] += chunk
-            for i in range(i0, min(i0 + chunk, len(inp["radius"]))):
-                O.full_dynamics([inp["radius"][i]], [23e3], [inp["anisotropy_axis"][i]],
-                                [inp["magnetisation_direction"][i]], [[0, 0, 0]], 400e3, 0.1,
-                                300.0, False, True, False, 1e-9, dt, end, 2, int(inp["seeds"][i]),
-                                field_shape='sine', field_amplitude=0.02 / MU0,
-                                field_frequency=3e5, rng="mt")
-                with lock:
-                    state["done"] += 1
+                i = state["next"] % n_members
+                state["next"] += 1
+            ps = member(i)
+            with lock:
+                state["done"] += ps
 
     def run(nt, secs):
         state["next"], state["done"] = 0, 0
@@ -90,16 +150,17 @@ def cpu_baseline(inp, seconds=12.0, steps=10000, chunk=16):
             t.start()
         for t in th:
             t.join()
-        el = time.perf_counter() - t0
-        return state["done"] * (steps + 1) / el, state["done"]
+        return state["done"] / (time.perf_counter() - t0), state["next"]
 
-    one, n1 = run(1, seconds / 4)
-    allc, na = run(n_threads, seconds)
+    one, m1 = run(1, seconds / 4)
+    allc, ma = run(n_threads, seconds)
+    what = {"ac_field": "Heun", "implicit": "implicit-midpoint", "clusters": "Heun, dipolar"}[name]
     return {"value": allc, "unit": "particle-steps/s", "cores": n_threads, "kind": "port",
             "value_1core": one,
-            "sample": "%d particles x %d Heun steps each (FP64 oracle, mt19937_64 Wiener "
-                      "draws, dense 3x3 diffusion, function-pointer sde) on %d threads; "
-                      "1-core rate from %d particles" % (na, steps + 1, n_threads, n1)}
+            "sample": "%d members x %d %s steps each (FP64 oracle in the reference's structure: "
+                      "mt19937_64 Wiener draws, dense 3N x 3N diffusion, function-pointer sde) "
+                      "on %d threads; 1-core rate from %d members" % (ma, steps + 1, what,
+                                                                     n_threads, m1)}
 
 
 def copy_calibration(nbytes):
@@ -124,9 +185,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--block", type=int, default=131072, help="time steps per bench step")
-    ap.add_argument("--particles", type=int, default=65536)
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--block", type=int, default=0, help="time steps per bench step")
+    ap.add_argument("--particles", type=int, default=0, help="particles (clusters for 'clusters')")
+    ap.add_argument("--workload", default="ac_field", choices=sorted(DEFAULTS))
+    ap.add_argument("--precision", default="")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
@@ -141,55 +203,60 @@ def main():
 
     import magpy_amd
     core = magpy_amd.core
-    n = args.particles
-    inp = workload(n)
-    cycles, f = 5, 3e5
-    end_time = cycles / f
-    max_samples = 2000 * cycles + 1
+    n = args.particles or DEFAULTS[args.workload][0]
+    block = args.block or DEFAULTS[args.workload][1]
+    inp, kw, flop_per_step, prec, desc = workload(args.workload, n,
+                                                  block * (args.warmup + args.steps) + 1)
+    prec = args.precision or prec
+    n_particles = int(inp["member_offsets"][-1])
     print("device:", core.device_info(0), flush=True)
-    os.environ["MAGPY_AMD_STEPS_PER_LAUNCH"] = str(args.block)
-    ses = core.Session(time_step=1e-14, end_time=end_time, max_samples=max_samples,
-                       interactions=False, use_implicit=False, precision=args.precision,
-                       trajectories=False, ensemble_sum=True, **inp)
-    for i in range(args.warmup):
-        ses.advance(args.block)
+    os.environ["MAGPY_AMD_STEPS_PER_LAUNCH"] = str(block)
+    ses = core.Session(precision=prec, trajectories=False, ensemble_sum=True, **kw, **inp)
+    for _ in range(args.warmup):
+        ses.advance(block)
     info0 = ses.info()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        ses.advance(args.block)
+    for _ in range(args.steps):
+        ses.advance(block)
     t1 = time.perf_counter()
     info1 = ses.info()
     elapsed = t1 - t0
-    psteps = float(n) * args.block * args.steps
-    value = psteps / elapsed
+    value = float(n_particles) * block * args.steps / elapsed
     launches = info1["launches"] - info0["launches"]
-    kms = info1["kernel_ms"] - info0["kernel_ms"]
-    avg_launch_s = kms / 1e3 / max(launches, 1)
-    flop_per_launch = FLOP_PER_PARTICLE_STEP * float(n) * (args.block * args.steps / max(launches, 1))
-    achieved = flop_per_launch / avg_launch_s / 1e12
+    avg_launch_s = (info1["kernel_ms"] - info0["kernel_ms"]) / 1e3 / max(launches, 1)
+    steps_per_launch = block * args.steps / max(launches, 1)
     st, step = ses.state()
-    ok = bool(np.all(np.isfinite(st)))
     line = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if args.precision == "fp32" else "f64", "data": "synthetic",
-        "config": {"workload": "65,536-particle AC-field Heun ensemble (20 mT, 300 kHz, "
-                               "dt=1e-14 s), step = %d time steps of all particles" % args.block,
-                   "particles": n, "time_steps_per_step": args.block,
-                   "run_length_steps": info1["n_steps"], "samples": max_samples,
-                   "precision": args.precision, "state_finite": ok, "step_index": int(step)},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": (TRAFFIC_BYTES_DEFAULT_LAUNCH
-                                 if (n, args.block, args.precision) == (65536, 131072, "fp32")
-                                 else None),
-                     "kernel": "heun_indep_kernel<float, sine>",
-                     "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
-                     "flop_per_particle_step": FLOP_PER_PARTICLE_STEP},
+        "dtype": "f32" if prec == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": desc + ", step = %d time steps of all particles" % block,
+                   "particles": n_particles, "members": len(inp["member_offsets"]) - 1,
+                   "time_steps_per_step": block, "run_length_steps": info1["n_steps"],
+                   "precision": prec, "state_finite": bool(np.all(np.isfinite(st))),
+                   "cluster_block": info1["cluster_block"],
+                   "step_index": int(step)},
     }
+    if flop_per_step is not None:
+        achieved = flop_per_step * steps_per_launch / avg_launch_s / 1e12
+        peak = PEAK_TFLOPS[prec]
+        default = (args.workload, n, block, prec) == ("ac_field", 65536, 131072, "fp32")
+        line["roofline"] = {
+            "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": TRAFFIC_BYTES_DEFAULT_LAUNCH if default else None,
+            "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
+            "flop_per_time_step": flop_per_step}
+    else:
+        fin = ses.finish()
+        line["roofline"] = {"bound": "valu", "achieved": None, "peak": PEAK_TFLOPS[prec],
+                            "unit": "TFLOP/s", "frac": None, "traffic": None,
+                            "avg_launch_ms": avg_launch_s * 1e3, "launches": launches}
+        line["newton_iters_per_step"] = fin["newton_iters"] / (n_particles * max(fin["n_steps"], 1))
+        line["newton_failures"] = fin["newton_failures"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(inp)
+        line["cpu_baseline"] = cpu_baseline(args.workload, inp)
         line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
 

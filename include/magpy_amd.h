@@ -141,9 +141,10 @@ int mp_session_create(const mp_ensemble_spec *spec, int want_trajectories, int w
 /* advance every member by up to n_steps (stops at the end of the run);
  * *steps_done (optional) = steps taken so far after the call */
 int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done);
-/* total steps of the run, and kernel launches / device milliseconds so far */
+/* total steps of the run, kernel launches / device milliseconds so far, and
+ * the block size of interacting clusters (0 without clusters) */
 int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                    double *kernel_ms);
+                    double *kernel_ms, int32_t *cluster_block);
 /* reduced state [P][3] and the step index it belongs to */
 int mp_session_state(mp_session *session, double *state, int64_t *step);
 /* run to the end and fill `out` as mp_ensemble_run would */

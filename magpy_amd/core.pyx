@@ -65,7 +65,7 @@ cdef extern from "magpy_amd.h" nogil:
                           mp_session **session)
     int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done)
     int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                        double *kernel_ms)
+                        double *kernel_ms, int32_t *cluster_block)
     int mp_session_state(mp_session *session, double *state, int64_t *step)
     int mp_session_finish(mp_session *session, mp_ensemble_out *out)
     void mp_session_destroy(mp_session *session)
@@ -382,8 +382,9 @@ cdef class Session:
     def info(self):
         cdef int64_t tot = 0, launches = 0
         cdef double ms = 0
-        _check(mp_session_info(self.s, &tot, &launches, &ms))
-        return {'n_steps': tot, 'launches': launches, 'kernel_ms': ms}
+        cdef int32_t cblock = 0
+        _check(mp_session_info(self.s, &tot, &launches, &ms, &cblock))
+        return {'n_steps': tot, 'launches': launches, 'kernel_ms': ms, 'cluster_block': cblock}
 
     def state(self):
         st = np.zeros((self.spec.P, 3))

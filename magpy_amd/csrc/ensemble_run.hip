@@ -18,6 +18,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "simulation.hpp"
@@ -570,6 +571,7 @@ struct mp_session {
     StreamGuard sg;
     bool want_traj = false, want_ens = false;
     int device = 0;
+    int cluster_block = 0;
     double kernel_ms = 0.0;
 };
 
@@ -689,17 +691,60 @@ void validate(const EnsembleSpec &sp)
 
 namespace {
 
-// Block size for interacting clusters: the smallest multiple of the wave
-// (>= 256 lanes, for barrier amortisation) that holds the largest cluster.
+// Block size for interacting clusters: the smallest power of two of at
+// least one wave that holds the largest cluster.
 int cluster_block_size(int64_t nmax)
 {
     if (const char *env = std::getenv("MAGPY_AMD_CLUSTER_BLOCK")) {
         const int v = std::atoi(env);
         if (v >= nmax && v % mp::kWave == 0 && v <= 1024) return v;
     }
-    for (int b = 256; b <= 1024; b *= 2)
+    for (int b = mp::kWave; b <= 1024; b *= 2)
         if (b >= nmax) return b;
     return 0;
+}
+
+// Autotuner of the cluster block size: for runs long enough to repay it,
+// time a few trial steps of the first schedule group with every candidate
+// block (powers of two from the smallest that holds the largest cluster up
+// to 1024 lanes) on throw-away copies of the state, and keep the fastest.
+// Results are cached per (largest cluster, slots, precision, steps-per-group
+// bucket) for the life of the process.
+int autotune_cluster_block(const simulation::EnsembleSpec &sp, const Plan &plan, int64_t nmax,
+                           hipStream_t s)
+{
+    static std::map<std::tuple<int64_t, int64_t, int>, int> cache;
+    const auto key = std::make_tuple(nmax, (int64_t)plan.perm.size(), (int)sp.precision);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const int64_t trial_steps = 64;
+    hipEvent_t e0, e1;
+    MP_HIP(hipEventCreate(&e0));
+    MP_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 0.f;
+    for (int b = cluster_block_size(nmax); b != 0 && b <= 1024; b *= 2) {
+        std::unique_ptr<GroupBase> g;
+        if (sp.precision == MP_PRECISION_FP32)
+            g.reset(new GroupRun<float>(sp, plan, 0, false, false, b));
+        else
+            g.reset(new GroupRun<double>(sp, plan, 0, false, false, b));
+        g->enqueue(4, s);  // warm the code object
+        MP_HIP(hipEventRecord(e0, s));
+        g->enqueue(4 + trial_steps, s);
+        MP_HIP(hipEventRecord(e1, s));
+        MP_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        MP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (best == 0 || ms < best_ms) {
+            best = b;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cache[key] = best;
+    return best;
 }
 
 mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool ens)
@@ -776,8 +821,12 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
             if (cluster_block == 0)
                 throw simulation::Error(MP_ERR_UNSUPPORTED,
                                         "clusters of more than 1024 particles are not supported");
+            const bool fixed = std::getenv("MAGPY_AMD_CLUSTER_BLOCK") != nullptr;
+            if (!fixed && sp.noise == MP_NOISE_PHILOX && plan.group_steps[0] >= 20000)
+                cluster_block = autotune_cluster_block(sp, plan, nmax, ss->sg.s);
         }
     }
+    ss->cluster_block = cluster_block;
     for (int g = 0; g < G; ++g) {
         if (sp.precision == MP_PRECISION_FP32 && sp.scheme == MP_SCHEME_HEUN)
             ss->groups.emplace_back(new GroupRun<float>(sp, plan, g, traj, ens, cluster_block));
@@ -1002,7 +1051,7 @@ int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done
 }
 
 int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                    double *kernel_ms)
+                    double *kernel_ms, int32_t *cluster_block)
 {
     return guarded([&] {
         if (!session) throw simulation::Error(MP_ERR_ARG, "session is required");
@@ -1014,6 +1063,7 @@ int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launch
         if (n_steps_total) *n_steps_total = tot;
         if (launches) *launches = l;
         if (kernel_ms) *kernel_ms = session->kernel_ms;
+        if (cluster_block) *cluster_block = session->cluster_block;
     });
 }
 

@@ -388,30 +388,44 @@ struct alignas(16) Vec4 {
     T x, y, z, w;
 };
 
-// Dipolar field at lane `self` from lanes [b, e) of its cluster except itself.
+// Dipolar field at lane `self` from lanes [b, e) of its cluster except itself,
+// summed in increasing j (the reference's order, lib/field.cpp:193-202).  One
+// loop with the self term masked: split loops j < i, j > i would give the
+// lanes of a wave complementary trip counts and run both in full.
 template <typename T>
 __device__ inline void dipolar_sum(const Vec4<T> *pos, const Vec4<T> *m, int self, int b, int e,
                                    T &hx, T &hy, T &hz)
 {
     const Vec4<T> pi = pos[self];
-    hx = hy = hz = T(0);
-    auto term = [&](int j) {
-        const Vec4<T> pj = pos[j];
-        const Vec4<T> mj = m[j];
+    T ax = T(0), ay = T(0), az = T(0);
+    auto term = [&](const Vec4<T> &pj, const Vec4<T> &mj, bool skip) {
         const T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        const T r2 = dx * dx + dy * dy + dz * dz;
+        const T r2 = skip ? T(1) : dx * dx + dy * dy + dz * dz;
         T rinv;
         if constexpr (sizeof(T) == 4) rinv = __builtin_amdgcn_rsqf(r2);
         else rinv = 1.0 / sqrt(r2);
         const T rinv2 = rinv * rinv;
-        const T c = pj.w * rinv2 * rinv;
+        const T c = skip ? T(0) : pj.w * rinv2 * rinv;
         const T t = T(3) * (mj.x * dx + mj.y * dy + mj.z * dz) * rinv2;
-        hx += c * (t * dx - mj.x);
-        hy += c * (t * dy - mj.y);
-        hz += c * (t * dz - mj.z);
+        ax += c * (t * dx - mj.x);
+        ay += c * (t * dy - mj.y);
+        az += c * (t * dz - mj.z);
     };
-    for (int j = b; j < self; ++j) term(j);
-    for (int j = self + 1; j < e; ++j) term(j);
+    int j = b;
+    for (; j + 4 <= e; j += 4) {
+        Vec4<T> pj[4], mj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pj[u] = pos[j + u];
+            mj[u] = m[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) term(pj[u], mj[u], j + u == self);
+    }
+    for (; j < e; ++j) term(pos[j], m[j], j == self);
+    hx = ax;
+    hy = ay;
+    hz = az;
 }
 
 template <typename T, int kShape, bool kTable, bool kRenorm>
@@ -441,81 +455,78 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
     keys.init(a.key0[p], a.key1[p]);
     const uint32_t pid = a.pid[p];
     const int64_t n0 = a.n0, n1 = a.n1;
+    const int64_t tab0 = 4 * (n0 >> 2);
     pos[tid] = Vec4<T>{ca.px[p], ca.py[p], ca.pz[p], ca.w[p]};
 
     int64_t s = a.s_next;
     int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
-    T z[4];
-    uint64_t have = UINT64_MAX;
 
-    for (int64_t n = n0; n < n1; ++n) {
-        T w3[3];
+    // the normals of step group g+1 are drawn while group g is integrated
+    T zc[12], zn[12];
+    const int64_t g_begin = n0 >> 2, g_end = (n1 + 3) >> 2;
+    group_normals(g_begin, pid, a.tag, keys, sig, zc);
+    for (int64_t g = g_begin; g < g_end; ++g) {
+        group_normals(g + 1, pid, a.tag, keys, sig, zn);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const uint64_t j = 3 * (uint64_t)n + d;
-            const uint64_t c = j >> 2;
-            if (c != have) {
-                const u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, a.tag}, keys);
-                Num<T>::box_muller(r.x, r.y, z[0], z[1]);
-                Num<T>::box_muller(r.z, r.w, z[2], z[3]);
-                have = c;
+        for (int u = 0; u < 4; ++u) {
+            const int64_t n = 4 * g + u;
+            if (n < n0 || n >= n1) continue;
+            const T sx = zc[3 * u], sy = zc[3 * u + 1], sz = zc[3 * u + 2];
+            T h1, h2;
+            if constexpr (kTable) {
+                h1 = (T)htab[n - tab0];
+                h2 = (T)htab[n - tab0 + 1];
+            } else if constexpr (sizeof(T) == 4) {
+                const double c1 = (double)(n + 1) * fdt;
+                h1 = Applied<T, kShape>::at(shape, amp, c1);
+                h2 = Applied<T, kShape>::at(shape, amp, c1 + fdt);
+            } else {
+                const double t1 = (double)(n + 1) * dtd;
+                h1 = Applied<T, kShape>::at(shape, amp, fred, t1);
+                h2 = Applied<T, kShape>::at(shape, amp, fred, t1 + dtd);
             }
-            const int jj = (int)(j & 3);
-            w3[d] = jj == 0 ? z[0] : jj == 1 ? z[1] : jj == 2 ? z[2] : z[3];
-        }
-        const T sx = sig * w3[0], sy = sig * w3[1], sz = sig * w3[2];
-        T h1, h2;
-        if constexpr (kTable) {
-            h1 = (T)htab[n - 4 * (n0 >> 2)];
-            h2 = (T)htab[n - 4 * (n0 >> 2) + 1];
-        } else if constexpr (sizeof(T) == 4) {
-            const double c1 = (double)(n + 1) * fdt;
-            h1 = Applied<T, kShape>::at(shape, amp, c1);
-            h2 = Applied<T, kShape>::at(shape, amp, c1 + fdt);
-        } else {
-            const double t1 = (double)(n + 1) * dtd;
-            h1 = Applied<T, kShape>::at(shape, amp, fred, t1);
-            h2 = Applied<T, kShape>::at(shape, amp, fred, t1 + dtd);
-        }
-        // stage 1
-        mb0[tid] = Vec4<T>{mx, my, mz, T(0)};
-        __syncthreads();
-        T hx, hy, hz;
-        dipolar_sum(pos, mb0, tid, cb, ce, hx, hy, hz);
-        T d = mx * kex + my * key + mz * kez;
-        T vx = d * edx + (hx * dt + sx), vy = d * edy + (hy * dt + sy),
-          vz = d * edz + ((hz + h1) * dt + sz);
-        T ax, ay, az;
-        llg_map(mx, my, mz, vx, vy, vz, al, ax, ay, az);
-        const T tx = mx + ax, ty = my + ay, tz = mz + az;
-        // stage 2
-        mb1[tid] = Vec4<T>{tx, ty, tz, T(0)};
-        __syncthreads();
-        dipolar_sum(pos, mb1, tid, cb, ce, hx, hy, hz);
-        d = tx * kex + ty * key + tz * kez;
-        vx = d * edx + (hx * dt + sx);
-        vy = d * edy + (hy * dt + sy);
-        vz = d * edz + ((hz + h2) * dt + sz);
-        T bx, by, bz;
-        llg_map(tx, ty, tz, vx, vy, vz, al, bx, by, bz);
-        mx += T(0.5) * (ax + bx);
-        my += T(0.5) * (ay + by);
-        mz += T(0.5) * (az + bz);
-        if constexpr (kRenorm) {
-            if (live) {
-                const T nrm = sqrt(mx * mx + my * my + mz * mz);
-                mx = mx / nrm;
-                my = my / nrm;
-                mz = mz / nrm;
+            // stage 1
+            mb0[tid] = Vec4<T>{mx, my, mz, T(0)};
+            __syncthreads();
+            T hx, hy, hz;
+            dipolar_sum(pos, mb0, tid, cb, ce, hx, hy, hz);
+            T d = mx * kex + my * key + mz * kez;
+            T vx = d * edx + (hx * dt + sx), vy = d * edy + (hy * dt + sy),
+              vz = d * edz + ((hz + h1) * dt + sz);
+            T ax, ay, az;
+            llg_map(mx, my, mz, vx, vy, vz, al, ax, ay, az);
+            const T tx = mx + ax, ty = my + ay, tz = mz + az;
+            // stage 2
+            mb1[tid] = Vec4<T>{tx, ty, tz, T(0)};
+            __syncthreads();
+            dipolar_sum(pos, mb1, tid, cb, ce, hx, hy, hz);
+            d = tx * kex + ty * key + tz * kez;
+            vx = d * edx + (hx * dt + sx);
+            vy = d * edy + (hy * dt + sy);
+            vz = d * edz + ((hz + h2) * dt + sz);
+            T bx, by, bz;
+            llg_map(tx, ty, tz, vx, vy, vz, al, bx, by, bz);
+            mx += T(0.5) * (ax + bx);
+            my += T(0.5) * (ay + by);
+            mz += T(0.5) * (az + bz);
+            if constexpr (kRenorm) {
+                if (live) {
+                    const T nrm = sqrt(mx * mx + my * my + mz * mz);
+                    mx = mx / nrm;
+                    my = my / nrm;
+                    mz = mz / nrm;
+                }
+            }
+            if (n + 1 == next_rec) {
+                do {
+                    record_sample(a, s, p, live, mx, my, mz, msw);
+                    ++s;
+                    next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+                } while (next_rec == n + 1);
             }
         }
-        if (n + 1 == next_rec) {
-            do {
-                record_sample(a, s, p, live, mx, my, mz, msw);
-                ++s;
-                next_rec = s < a.S ? a.sched[s] : INT64_MAX;
-            } while (next_rec == n + 1);
-        }
+#pragma unroll
+        for (int i = 0; i < 12; ++i) zc[i] = zn[i];
     }
     a.mx[p] = mx;
     a.my[p] = my;
