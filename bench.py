@@ -48,7 +48,8 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (published)
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) scaled against a 1 GiB
 # device copy; see profiles/heun_indep_fp32_65536_ac_field.md.
 TRAFFIC_BYTES_DEFAULT_LAUNCH = 13.30e6
-DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256)}
+DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256),
+            "replicas": (4096, 64)}
 
 
 def ac_field_inputs(n, seed=2024):
@@ -76,6 +77,21 @@ def cluster_inputs(n_clusters, seed=2025):
     return e
 
 
+def replica_inputs(n_replicas, n=512, seed=2026):
+    from magpy_amd import geometry as G
+    rng = np.random.default_rng(seed)
+    loc = G.random_cluster(n, 10e-9, 0.15, 2.2, rng)
+    axes = G.uniform_random_axes(n, rng)
+    P = n * n_replicas
+    return dict(member_offsets=np.arange(n_replicas + 1, dtype=np.int64) * n,
+                radius=np.full(P, 10e-9), anisotropy=np.full(P, 23e3),
+                anisotropy_axis=np.tile(axes, (n_replicas, 1)),
+                magnetisation_direction=np.tile(axes, (n_replicas, 1)),
+                location=np.tile(loc, (n_replicas, 1)), magnetisation=400e3, damping=0.1,
+                temperature=300.0, field_shape='sine', field_amplitude=0.02 / MU0,
+                field_frequency=3e5, seeds=rng.integers(0, 2 ** 31 - 1, size=n_replicas))
+
+
 def workload(name, n, timed_steps):
     """(inputs, session kwargs, algorithmic FLOP per time step, precision, description)."""
     if name == "ac_field":
@@ -100,6 +116,16 @@ def workload(name, n, timed_steps):
         return (inp, kw, flop, "fp32",
                 "4,096 interacting clusters N~U{2..64}, 10% volume fraction, Heun FP32 "
                 "(dt=1e-14 s, 2e5 steps)")
+    if name == "replicas":
+        inp = replica_inputs(n)
+        n3 = 3 * 512
+        flop = float(2 * 2 * n3 * n3 * n + FLOP_PER_PARTICLE_STEP * 512 * n)
+        kw = dict(time_step=1e-14, end_time=1e5 * 1e-14, max_samples=101, interactions=True,
+                  use_implicit=False)
+        return (inp, kw, flop, "fp32",
+                "one 512-particle cluster (15% volume fraction) x 4,096 thermal replicas, "
+                "dipolar field as a BF16x3 MFMA product, Heun FP32 (20 mT, 300 kHz, "
+                "dt=1e-14 s, 1e5 steps)")
     raise SystemExit("unknown workload " + name)
 
 
@@ -112,6 +138,8 @@ def cpu_baseline(name, inp, seconds=12.0):
     n_members = len(offs) - 1
     if name == "ac_field":
         dt, steps, implicit, interactions = 1e-14, 10000, False, False
+    elif name == "replicas":
+        dt, steps, implicit, interactions = 1e-14, 20, False, True
     elif name == "implicit":
         dt, steps, implicit, interactions = 5e-13, 100, True, False
     else:
@@ -154,13 +182,47 @@ def cpu_baseline(name, inp, seconds=12.0):
 
     one, m1 = run(1, seconds / 4)
     allc, ma = run(n_threads, seconds)
-    what = {"ac_field": "Heun", "implicit": "implicit-midpoint", "clusters": "Heun, dipolar"}[name]
+    what = {"ac_field": "Heun", "implicit": "implicit-midpoint", "clusters": "Heun, dipolar",
+            "replicas": "Heun, dipolar (512 particles)"}[name]
     return {"value": allc, "unit": "particle-steps/s", "cores": n_threads, "kind": "port",
             "value_1core": one,
             "sample": "%d members x %d %s steps each (FP64 oracle in the reference's structure: "
                       "mt19937_64 Wiener draws, dense 3N x 3N diffusion, function-pointer sde) "
                       "on %d threads; 1-core rate from %d members" % (ma, steps + 1, what,
                                                                      n_threads, m1)}
+
+
+def sar_run(n, precision, save):
+    """The full AC-field workload (5 cycles) in one precision; prints progress
+    and the final-cycle hysteresis energy through EnsembleResults."""
+    import magpy_amd
+    from magpy_amd.results import EnsembleResults
+    core = magpy_amd.core
+    inp = ac_field_inputs(n)
+    f, cycles = 3e5, 5
+    print("device:", core.device_info(0), flush=True)
+    ses = core.Session(precision=precision, trajectories=False, ensemble_sum=True,
+                       time_step=1e-14, end_time=cycles / f, max_samples=2000 * cycles + 1,
+                       interactions=False, use_implicit=False, **inp)
+    total = ses.info()["n_steps"]
+    t0 = time.perf_counter()
+    done = 0
+    chunk = 1 << 24
+    while done < total:
+        done = ses.advance(chunk)
+        print(json.dumps({"progress_steps": int(done), "of": int(total),
+                          "elapsed_s": round(time.perf_counter() - t0, 1)}), flush=True)
+    res = ses.finish()
+    el = time.perf_counter() - t0
+    er = EnsembleResults.from_batch(res, inp["member_offsets"])
+    energy = er.final_cycle_energy_dissipated(f)
+    if save:
+        np.savez(save, time=res["time"], field=res["field"], ensemble_m=res["ensemble_m"],
+                 n_members=n)
+    print(json.dumps({"sar_run": precision, "particles": n, "steps": int(res["n_steps"]),
+                      "final_cycle_energy_J_per_m3": energy, "sar_W_per_m3": energy * f,
+                      "wall_s": el, "kernel_ms": res["kernel_ms"],
+                      "nonfinite": int(np.sum(res["status"] != 0))}), flush=True)
 
 
 def copy_calibration(nbytes):
@@ -188,13 +250,21 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="time steps per bench step")
     ap.add_argument("--particles", type=int, default=0, help="particles (clusters for 'clusters')")
     ap.add_argument("--workload", default="ac_field", choices=sorted(DEFAULTS))
+    ap.add_argument("--dipolar", default="auto",
+                    choices=["auto", "pairs", "gemm_bf16x3", "gemm_f32"])
     ap.add_argument("--precision", default="")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one device copy of BYTES (counter calibration)")
+    ap.add_argument("--sar-run", default="", choices=["", "fp32", "fp64"],
+                    help="run the whole AC-field workload in this precision and report SAR")
+    ap.add_argument("--save", default="", help="npz file for the --sar-run time series")
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
+        return
+    if args.sar_run:
+        sar_run(args.particles or 65536, args.sar_run, args.save)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build's path runs on a single MI355X "
@@ -211,7 +281,8 @@ def main():
     n_particles = int(inp["member_offsets"][-1])
     print("device:", core.device_info(0), flush=True)
     os.environ["MAGPY_AMD_STEPS_PER_LAUNCH"] = str(block)
-    ses = core.Session(precision=prec, trajectories=False, ensemble_sum=True, **kw, **inp)
+    ses = core.Session(precision=prec, trajectories=False, ensemble_sum=True,
+                       dipolar=args.dipolar, **kw, **inp)
     for _ in range(args.warmup):
         ses.advance(block)
     info0 = ses.info()
@@ -235,10 +306,22 @@ def main():
                    "particles": n_particles, "members": len(inp["member_offsets"]) - 1,
                    "time_steps_per_step": block, "run_length_steps": info1["n_steps"],
                    "precision": prec, "state_finite": bool(np.all(np.isfinite(st))),
-                   "cluster_block": info1["cluster_block"],
+                   "cluster_block": info1["cluster_block"], "dipolar": info1["dipolar"],
                    "step_index": int(step)},
     }
-    if flop_per_step is not None:
+    if flop_per_step is not None and args.workload == "replicas":
+        # two products of (3N)^2 x R per step dominate; BF16 three-term split
+        # runs three MFMA products for each
+        achieved = flop_per_step * steps_per_launch / avg_launch_s / 1e12
+        gemm_mode = info1["dipolar"]
+        peak = 2500.0 if gemm_mode == "gemm_bf16x3" else PEAK_TFLOPS["fp32"]
+        passes = 3 if gemm_mode == "gemm_bf16x3" else 1
+        line["roofline"] = {
+            "bound": "mfma" if gemm_mode.startswith("gemm") else "valu", "achieved": achieved,
+            "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "mfma_utilisation": passes * achieved / peak, "traffic": None,
+            "avg_step_ms": avg_launch_s * 1e3, "flop_per_time_step": flop_per_step}
+    elif flop_per_step is not None:
         achieved = flop_per_step * steps_per_launch / avg_launch_s / 1e12
         peak = PEAK_TFLOPS[prec]
         default = (args.workload, n, block, prec) == ("ac_field", 65536, 131072, "fp32")

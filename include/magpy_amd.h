@@ -47,6 +47,18 @@ extern "C" {
 #define MP_NOISE_PHILOX 0
 #define MP_NOISE_ARRAY 1
 
+/* Dipolar field of interacting clusters.  PAIRS: per-lane pair sums of each
+ * particle's cluster (any ensemble).  GEMM_BF16X3 / GEMM_F32: when every
+ * member is the same cluster (positions, radii, anisotropy, axes, material,
+ * field; only seeds and initial states differ), all replicas' fields at one
+ * stage are one product Mt D^T on MFMA, in BF16 with the three-term split and
+ * FP32 accumulation, or in FP32.  AUTO: GEMM_BF16X3 for FP32 Heun replica
+ * ensembles of >= 64 members with >= 32 particles, PAIRS otherwise. */
+#define MP_DIPOLAR_AUTO 0
+#define MP_DIPOLAR_PAIRS 1
+#define MP_DIPOLAR_GEMM_BF16X3 2
+#define MP_DIPOLAR_GEMM_F32 3
+
 /* per-particle status bits */
 #define MP_STATUS_NONFINITE 1u      /* state became NaN/Inf                     */
 #define MP_STATUS_NEWTON_FAIL 2u    /* an implicit step's Newton did not converge */
@@ -83,6 +95,13 @@ typedef struct mp_ensemble_spec {
     int64_t noise_steps;
     uint32_t noise_tag;
     int32_t device;                 /* HIP device ordinal */
+    int32_t dipolar;                /* MP_DIPOLAR_*: how interacting clusters sum the field */
+    /* Resume: start at step start_step from the reduced state start_state
+     * [P][3] (NULL: magnetisation_direction).  The Wiener stream is keyed by
+     * the step index, so a run split at any step and resumed gives the same
+     * result.  Samples taken before start_step are returned as NaN. */
+    int64_t start_step;
+    const double *start_state;
 } mp_ensemble_spec;
 
 typedef struct mp_ensemble_out {
@@ -141,10 +160,11 @@ int mp_session_create(const mp_ensemble_spec *spec, int want_trajectories, int w
 /* advance every member by up to n_steps (stops at the end of the run);
  * *steps_done (optional) = steps taken so far after the call */
 int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done);
-/* total steps of the run, kernel launches / device milliseconds so far, and
- * the block size of interacting clusters (0 without clusters) */
+/* total steps of the run, kernel launches / device milliseconds so far, the
+ * block size of interacting clusters (0 without pair sums) and the
+ * MP_DIPOLAR_* path the run uses (0 without interacting clusters) */
 int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                    double *kernel_ms, int32_t *cluster_block);
+                    double *kernel_ms, int32_t *cluster_block, int32_t *dipolar);
 /* reduced state [P][3] and the step index it belongs to */
 int mp_session_state(mp_session *session, double *state, int64_t *step);
 /* run to the end and fill `out` as mp_ensemble_run would */
@@ -155,6 +175,17 @@ void mp_session_destroy(mp_session *session);
  * k[s] = step index recorded as sample s; *n_steps = steps the loop takes. */
 int mp_sample_schedule(double dt_reduced, double end_reduced, int64_t max_samples,
                        int64_t *k, int64_t *n_steps);
+
+/* Reduced dipolar field h[r][3i+a] of `reps` states m[r][3j+b] of one
+ * cluster (positions, radii, anisotropy, Ms as simulation::full_dynamics
+ * takes them), computed on the device as the product of the dipole tensor
+ * with the states: mode MP_DIPOLAR_GEMM_BF16X3, MP_DIPOLAR_GEMM_F32, or
+ * MP_DIPOLAR_PAIRS (FP64 product on the vector ALU).  Exposes the replica
+ * path's product for accuracy checks; the reference computes the same field
+ * per member in field::multi_add_dipolar (lib/field.cpp:187-203). */
+int mp_dipolar_field(const double *location, const double *radius, const double *anisotropy,
+                     size_t n_particles, double magnetisation, const double *m, size_t reps,
+                     int mode, double *h, int device);
 
 /* Fill out[n] with normals j0..j0+n-1 of one particle's Wiener stream
  * (MP_NOISE_PHILOX definition above) computed on the device in the given

@@ -41,6 +41,9 @@ cdef extern from "magpy_amd.h" nogil:
         int64_t noise_steps
         uint32_t noise_tag
         int32_t device
+        int32_t dipolar
+        int64_t start_step
+        const double *start_state
 
     ctypedef struct mp_ensemble_out:
         double *time
@@ -65,7 +68,10 @@ cdef extern from "magpy_amd.h" nogil:
                           mp_session **session)
     int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done)
     int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                        double *kernel_ms, int32_t *cluster_block)
+                        double *kernel_ms, int32_t *cluster_block, int32_t *dipolar)
+    int mp_dipolar_field(const double *location, const double *radius, const double *anisotropy,
+                         size_t n_particles, double magnetisation, const double *m, size_t reps,
+                         int mode, double *h, int device)
     int mp_session_state(mp_session *session, double *state, int64_t *step)
     int mp_session_finish(mp_session *session, mp_ensemble_out *out)
     void mp_session_destroy(mp_session *session)
@@ -83,6 +89,7 @@ cdef extern from "magpy_amd.h" nogil:
     int MP_SCHEME_HEUN, MP_SCHEME_IMPLICIT_MIDPOINT
     int MP_PRECISION_FP32, MP_PRECISION_FP64
     int MP_NOISE_PHILOX, MP_NOISE_ARRAY
+    int MP_DIPOLAR_AUTO, MP_DIPOLAR_PAIRS, MP_DIPOLAR_GEMM_BF16X3, MP_DIPOLAR_GEMM_F32
 
 
 class MagpyAMDError(RuntimeError):
@@ -99,6 +106,9 @@ class UnsupportedError(MagpyAMDError):
 
 FIELD_SHAPES = {'sine': MP_FIELD_SINE, 'square': MP_FIELD_SQUARE, 'constant': MP_FIELD_CONSTANT}
 PRECISIONS = {'fp32': MP_PRECISION_FP32, 'fp64': MP_PRECISION_FP64}
+DIPOLAR = {'auto': MP_DIPOLAR_AUTO, 'pairs': MP_DIPOLAR_PAIRS,
+           'gemm_bf16x3': MP_DIPOLAR_GEMM_BF16X3, 'gemm_f32': MP_DIPOLAR_GEMM_F32}
+DIPOLAR_NAMES = {v: k for k, v in DIPOLAR.items()}
 
 
 def _check(int rc):
@@ -152,6 +162,31 @@ def sample_schedule(double dt_reduced, double end_reduced, int64_t max_samples):
     return k, n_steps
 
 
+def dipolar_field(location, radius, anisotropy, double magnetisation, m, mode='gemm_bf16x3',
+                  int device=0):
+    """Reduced dipolar field of states m [reps, N, 3] of one cluster, from the
+    device product of the dipole tensor with the states (mode 'gemm_bf16x3',
+    'gemm_f32', or 'pairs' for the FP64 product).  Returns [reps, N, 3]."""
+    loc = _c(np.reshape(location, (-1, 3)), np.float64)
+    cdef size_t n = loc.shape[0]
+    r = _c(radius, np.float64)
+    k = _c(anisotropy, np.float64)
+    mm = _c(np.reshape(m, (-1, n, 3)), np.float64)
+    cdef size_t reps = mm.shape[0]
+    h = np.zeros((reps, n, 3))
+    cdef const double[:, ::1] loc_v = loc
+    cdef const double[::1] r_v = r, k_v = k
+    cdef const double[:, :, ::1] m_v = mm
+    cdef double[:, :, ::1] h_v = h
+    cdef int md = DIPOLAR[mode]
+    cdef int rc
+    with nogil:
+        rc = mp_dipolar_field(&loc_v[0, 0], &r_v[0], &k_v[0], n, magnetisation, &m_v[0, 0, 0],
+                              reps, md, &h_v[0, 0, 0], device)
+    _check(rc)
+    return h
+
+
 def wiener_normals(uint64_t seed, uint32_t particle, uint64_t j0, size_t n, uint32_t tag=0,
                    precision='fp64', int device=0):
     """Normals j0..j0+n-1 of one particle's Philox Wiener stream, generated on the device."""
@@ -176,7 +211,8 @@ cdef class _Spec:
                  field_shape, field_amplitude, field_frequency, seeds, double time_step,
                  double end_time, int64_t max_samples, bint renorm=False, bint interactions=True,
                  bint use_implicit=False, double implicit_tol=1e-9, int implicit_max_iter=1000,
-                 precision='fp64', noise=None, uint32_t noise_tag=0, int device=0):
+                 precision='fp64', noise=None, uint32_t noise_tag=0, int device=0,
+                 dipolar='auto', int64_t start_step=0, start_state=None):
         if max_samples < 1:
             raise ValueError("max_samples must be >= 1 (-1 never terminates in the reference loop)")
         offs = _c(member_offsets, np.int64)
@@ -203,7 +239,10 @@ cdef class _Spec:
             nz = _c(noise, np.float64)
             if nz.ndim != 3 or nz.shape[1] != P or nz.shape[2] != 3:
                 raise ValueError("noise must have shape [steps, P, 3]")
-        self.arrays = (offs, r, k, ax, m0, loc, ms, al, tk, fs, fa, ff, sd, nz)
+        st0 = None
+        if start_state is not None:
+            st0 = _c(np.reshape(start_state, (P, 3)), np.float64)
+        self.arrays = (offs, r, k, ax, m0, loc, ms, al, tk, fs, fa, ff, sd, nz, st0)
         self.M, self.P, self.S = M, P, max_samples
 
         cdef const int64_t[::1] offs_v = offs
@@ -238,6 +277,14 @@ cdef class _Spec:
         self.sp.implicit_max_iter = implicit_max_iter
         self.sp.noise_tag = noise_tag
         self.sp.device = device
+        self.sp.dipolar = DIPOLAR[dipolar]
+        self.sp.start_step = start_step
+        cdef const double[:, ::1] st_v
+        if st0 is not None:
+            st_v = st0
+            self.sp.start_state = &st_v[0, 0]
+        else:
+            self.sp.start_state = NULL
         if nz is not None:
             nz_v = nz.reshape(-1)
             self.sp.noise = MP_NOISE_ARRAY
@@ -312,7 +359,8 @@ def simulate_ensemble(member_offsets, radius, anisotropy, anisotropy_axis,
                       bint renorm=False, bint interactions=True, bint use_implicit=False,
                       double implicit_tol=1e-9, int implicit_max_iter=1000,
                       precision='fp64', trajectories=True, ensemble_sum=True,
-                      noise=None, uint32_t noise_tag=0, int device=0, member_times=False):
+                      noise=None, uint32_t noise_tag=0, int device=0, member_times=False,
+                      dipolar='auto', int64_t start_step=0, start_state=None):
     """Run every member of an ensemble in one batched device call.
 
     Per-particle arrays are concatenated over members in member order;
@@ -323,14 +371,17 @@ def simulate_ensemble(member_offsets, radius, anisotropy, anisotropy_axis,
     and particles of Ms*m) when ``ensemble_sum``, 'final_state' [P,3],
     'status' [P], 'n_steps', 'newton_iters', 'newton_failures', 'kernel_ms'.
     ``noise`` (optional) is an array [steps, P, 3] of Wiener draws replacing
-    the Philox stream.
+    the Philox stream.  ``start_step``/``start_state`` [P,3] resume a run at
+    that step from a 'final_state' or Session.state() of an earlier run of
+    the same ensemble (the Philox stream is indexed by step, so a split run
+    equals the unsplit one); samples before start_step come back as NaN.
     """
     cdef _Spec spec = _Spec(member_offsets, radius, anisotropy, anisotropy_axis,
                             magnetisation_direction, location, magnetisation, damping,
                             temperature, field_shape, field_amplitude, field_frequency, seeds,
                             time_step, end_time, max_samples, renorm, interactions,
                             use_implicit, implicit_tol, implicit_max_iter, precision, noise,
-                            noise_tag, device)
+                            noise_tag, device, dipolar, start_step, start_state)
     cdef mp_ensemble_out out
     res = _outputs(&out, spec.M, spec.P, spec.S, trajectories, ensemble_sum, member_times)
     cdef int rc
@@ -382,9 +433,10 @@ cdef class Session:
     def info(self):
         cdef int64_t tot = 0, launches = 0
         cdef double ms = 0
-        cdef int32_t cblock = 0
-        _check(mp_session_info(self.s, &tot, &launches, &ms, &cblock))
-        return {'n_steps': tot, 'launches': launches, 'kernel_ms': ms, 'cluster_block': cblock}
+        cdef int32_t cblock = 0, dip = 0
+        _check(mp_session_info(self.s, &tot, &launches, &ms, &cblock, &dip))
+        return {'n_steps': tot, 'launches': launches, 'kernel_ms': ms, 'cluster_block': cblock,
+                'dipolar': DIPOLAR_NAMES.get(dip, 'none') if dip else 'none'}
 
     def state(self):
         st = np.zeros((self.spec.P, 3))

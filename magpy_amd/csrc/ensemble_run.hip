@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <new>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "simulation.hpp"
+#include "replica_kernels.hpp"
 #include "sllg_kernels.hpp"
 
 namespace {
@@ -188,7 +190,7 @@ struct Plan {
 
 // Host copies of what the outputs need after the caller's spec is gone.
 struct HostCopy {
-    int64_t M = 0, P = 0, S = 0;
+    int64_t M = 0, P = 0, S = 0, start = 0;   // m0 is the state at step `start`
     std::vector<double> m0, ms;
     std::vector<int32_t> shape;
 };
@@ -297,6 +299,7 @@ struct GroupRun : GroupBase {
         want_traj = traj;
         want_ens = ens;
         spl = steps_per_launch(sp.scheme, sp.precision);
+        n_done = std::min(sp.start_step, n_total);
         cluster = cluster_block > 0;
         std::vector<int> cbase, csize;
         if (cluster) {
@@ -329,9 +332,10 @@ struct GroupRun : GroupBase {
             ex[i] = sp.anisotropy_axis[3 * p];
             ey[i] = sp.anisotropy_axis[3 * p + 1];
             ez[i] = sp.anisotropy_axis[3 * p + 2];
-            mx[i] = sp.magnetisation_direction[3 * p];
-            my[i] = sp.magnetisation_direction[3 * p + 1];
-            mz[i] = sp.magnetisation_direction[3 * p + 2];
+            const double *m0 = sp.start_state ? sp.start_state : sp.magnetisation_direction;
+            mx[i] = m0[3 * p];
+            my[i] = m0[3 * p + 1];
+            mz[i] = m0[3 * p + 2];
             k[i] = r.k_red[(size_t)lp];
             al[i] = sp.damping[m];
             dt[i] = r.dt;
@@ -562,6 +566,278 @@ struct GroupRun : GroupBase {
     }
 };
 
+// ------------------------------------------------------------------ replicas
+int pad_to(int64_t x, int64_t q) { return (int)((x + q - 1) / q * q); }
+
+// Dipole tensor of one cluster in reduced units (lib/field.cpp:187-225,
+// lib/simulation.cpp:577-583): D[3i+a][3j+b] = w_j r^-3 (3 u_a u_b - d_ab),
+// w_j = mu0 Ms^2 / (8 pi K_avg) v_j / V_avg, row-major with leading dimension
+// ld, zero padded.
+void dipole_tensor(const simulation::ReducedMember &r, const double *loc, double ms, size_t n,
+                   int ld, std::vector<double> &D)
+{
+    const double vroot = std::pow(r.average_volume, 1. / 3);
+    const double pre = kMU0 * ms * ms / 8.0 / kPiHost / r.average_anisotropy;
+    D.assign((size_t)ld * ld, 0.0);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) {
+            if (i == j) continue;
+            double d[3], rr = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                d[a] = (loc[3 * j + a] - loc[3 * i + a]) / vroot;
+                rr += d[a] * d[a];
+            }
+            rr = std::sqrt(rr);
+            const double c = pre * r.v_red[j] / (rr * rr * rr);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    D[(3 * i + a) * (size_t)ld + 3 * j + b] =
+                        c * (3 * (d[a] / rr) * (d[b] / rr) - (a == b ? 1.0 : 0.0));
+        }
+}
+
+void split_host(const std::vector<double> &x, std::vector<__bf16> &hi, std::vector<__bf16> &lo)
+{
+    hi.resize(x.size());
+    lo.resize(x.size());
+    for (size_t i = 0; i < x.size(); ++i) {
+        const float f = (float)x[i];
+        hi[i] = (__bf16)f;
+        lo[i] = (__bf16)(f - (float)hi[i]);
+    }
+}
+
+void launch_product(int mode, const float *a32, const __bf16 *a_hi, const __bf16 *a_lo,
+                    const float *b32, const __bf16 *b_hi, const __bf16 *b_lo, float *c, int rows,
+                    int ld, hipStream_t s)
+{
+    const dim3 grid((unsigned)(rows / mp::kGemmTile), (unsigned)(ld / mp::kGemmTile));
+    if (mode == MP_DIPOLAR_GEMM_F32)
+        hipLaunchKernelGGL(mp::gemm_f32_kernel, grid, dim3(256), 0, s, a32, b32, c, ld, ld);
+    else
+        hipLaunchKernelGGL(mp::gemm_bf16x3_kernel, grid, dim3(256), 0, s, a_hi, a_lo, b_hi, b_lo,
+                           c, ld, ld);
+}
+
+// All members are one cluster replicated: state [rows][ld] replica-major,
+// four launches per Heun step (product, stage 1, product, stage 2).
+struct ReplicaRun : GroupBase {
+    int n = 0, reps = 0, ld = 0, rows = 0, mode = MP_DIPOLAR_GEMM_BF16X3, n_waves = 0;
+    int shape = MP_FIELD_CONSTANT;
+    double happ = 0.0, fapp = 0.0, dt = 0.0;
+    bool want_traj = false, want_ens = false;
+    int64_t S = 0, s_next = 0;
+    std::vector<int64_t> sched, slot_particle;
+    DevBuf<float> d_m, d_mt, d_delta, d_h, d_kex, d_key, d_kez, d_edx, d_edy, d_edz, d_sig, d_b32,
+        d_samples;
+    DevBuf<__bf16> d_mhi, d_mlo, d_bhi, d_blo;
+    DevBuf<uint32_t> d_key0, d_key1, d_status;
+    DevBuf<double> d_msw, d_partial, d_ens;
+    DevBuf<int64_t> d_sched;
+    mp::ReplicaArgs a{};
+
+    ReplicaRun(const simulation::EnsembleSpec &sp, const Plan &plan, bool traj, bool ens, int m)
+    {
+        mode = m;
+        reps = (int)sp.n_members;
+        n = (int)(sp.member_offsets[1] - sp.member_offsets[0]);
+        ld = pad_to(3 * (int64_t)n, mp::kGemmTile);
+        rows = pad_to(reps, mp::kGemmTile);
+        sched = plan.sched[0];
+        S = (int64_t)sched.size();
+        n_total = plan.group_steps[0];
+        want_traj = traj;
+        want_ens = ens;
+        const simulation::ReducedMember &r = plan.red[0];
+        shape = sp.field_shape[0];
+        happ = r.happ;
+        fapp = r.fapp;
+        dt = r.dt;
+        slot_particle.resize((size_t)reps * n);
+        for (int rr = 0; rr < reps; ++rr)
+            for (int p = 0; p < n; ++p)
+                slot_particle[(size_t)rr * n + p] = sp.member_offsets[rr] + p;
+        // per-particle parameters (identical across replicas)
+        std::vector<double> kex(n), key(n), kez(n), edx(n), edy(n), edz(n), sig(n);
+        for (int p = 0; p < n; ++p) {
+            const double *e = sp.anisotropy_axis + 3 * p;
+            kex[p] = r.k_red[p] * e[0];
+            key[p] = r.k_red[p] * e[1];
+            kez[p] = r.k_red[p] * e[2];
+            edx[p] = e[0] * r.dt;
+            edy[p] = e[1] * r.dt;
+            edz[p] = e[2] * r.dt;
+            sig[p] = r.sr[p] * std::sqrt(r.dt);
+        }
+        d_kex.upload(cast_vec<float>(kex));
+        d_key.upload(cast_vec<float>(key));
+        d_kez.upload(cast_vec<float>(kez));
+        d_edx.upload(cast_vec<float>(edx));
+        d_edy.upload(cast_vec<float>(edy));
+        d_edz.upload(cast_vec<float>(edz));
+        d_sig.upload(cast_vec<float>(sig));
+        std::vector<uint32_t> k0(reps), k1(reps);
+        std::vector<double> msw(reps);
+        for (int rr = 0; rr < reps; ++rr) {
+            k0[rr] = (uint32_t)(uint64_t)sp.seed[rr];
+            k1[rr] = (uint32_t)((uint64_t)sp.seed[rr] >> 32);
+            msw[rr] = sp.magnetisation[rr];
+        }
+        d_key0.upload(k0);
+        d_key1.upload(k1);
+        d_msw.upload(msw);
+        // dipole tensor and its split
+        std::vector<double> D;
+        dipole_tensor(r, sp.location, sp.magnetisation[0], (size_t)n, ld, D);
+        if (mode == MP_DIPOLAR_GEMM_F32) {
+            d_b32.upload(cast_vec<float>(D));
+        } else {
+            std::vector<__bf16> hi, lo;
+            split_host(D, hi, lo);
+            d_bhi.upload(hi);
+            d_blo.upload(lo);
+        }
+        // state, replica-major and zero padded
+        std::vector<double> m0((size_t)rows * ld, 0.0);
+        const double *src = sp.start_state ? sp.start_state : sp.magnetisation_direction;
+        for (int rr = 0; rr < reps; ++rr)
+            for (int c = 0; c < 3 * n; ++c)
+                m0[(size_t)rr * ld + c] = src[3 * sp.member_offsets[rr] + c];
+        n_done = std::min(sp.start_step, n_total);
+        d_m.upload(cast_vec<float>(m0));
+        d_mt.upload(std::vector<float>(m0.size(), 0.f));
+        d_delta.upload(std::vector<float>(m0.size(), 0.f));
+        d_h.upload(std::vector<float>(m0.size(), 0.f));
+        d_mhi.alloc(m0.size());
+        d_mlo.alloc(m0.size());
+        hipLaunchKernelGGL(mp::split_kernel, dim3((unsigned)((m0.size() + 255) / 256)), dim3(256),
+                           0, 0, d_m.p, d_mhi.p, d_mlo.p, (int64_t)m0.size());
+        MP_HIP(hipGetLastError());
+        MP_HIP(hipDeviceSynchronize());
+        d_status.upload(std::vector<uint32_t>((size_t)reps * n, 0));
+        d_sched.upload(sched);
+        const int threads = reps * n;
+        const int grid = (threads + 255) / 256;
+        n_waves = grid * 4;
+        if (want_traj) d_samples.alloc((size_t)S * 3 * (size_t)threads);
+        if (want_ens) {
+            d_partial.alloc((size_t)S * 3 * (size_t)n_waves);
+            d_ens.alloc((size_t)S * 3);
+        }
+        a.n = n;
+        a.reps = reps;
+        a.ld = ld;
+        a.n_waves = n_waves;
+        a.kex = d_kex.p; a.key = d_key.p; a.kez = d_kez.p;
+        a.edx = d_edx.p; a.edy = d_edy.p; a.edz = d_edz.p;
+        a.sig = d_sig.p;
+        a.dt = (float)r.dt;
+        a.alpha = (float)sp.damping[0];
+        a.key0 = d_key0.p; a.key1 = d_key1.p;
+        a.msw = d_msw.p;
+        a.tag = sp.noise_tag;
+        a.renorm = sp.renorm ? 1 : 0;
+        a.m = d_m.p; a.mt = d_mt.p; a.delta = d_delta.p; a.h = d_h.p;
+        a.m_hi = d_mhi.p; a.m_lo = d_mlo.p;
+        a.status = d_status.p;
+        a.sched = d_sched.p;
+        a.S = S;
+        a.samples = want_traj ? d_samples.p : nullptr;
+        a.ens_partial = want_ens ? d_partial.p : nullptr;
+    }
+
+    const std::vector<int64_t> &slots() const override { return slot_particle; }
+
+    void enqueue(int64_t upto, hipStream_t s) override
+    {
+        upto = std::min(upto, n_total);
+        const int threads = reps * n;
+        const dim3 grid((unsigned)((threads + 255) / 256));
+        while (n_done < upto) {
+            const int64_t st = n_done;
+            const double t1 = (double)(st + 1) * dt;
+            const float h1 = (float)applied_host(shape, happ, fapp, t1);
+            const float h2 = (float)applied_host(shape, happ, fapp, t1 + dt);
+            launch_product(mode, d_m.p, d_mhi.p, d_mlo.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows,
+                           ld, s);
+            hipLaunchKernelGGL(mp::replica_stage1_kernel, grid, dim3(256), 0, s, a, st, h1);
+            launch_product(mode, d_mt.p, d_mhi.p, d_mlo.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows,
+                           ld, s);
+            while (s_next < S && sched[(size_t)s_next] < st + 1) ++s_next;
+            int n_rec = 0;
+            while (s_next + n_rec < S && sched[(size_t)(s_next + n_rec)] == st + 1) ++n_rec;
+            hipLaunchKernelGGL(mp::replica_stage2_kernel, grid, dim3(256), 0, s, a, st, h2,
+                               s_next, n_rec);
+            MP_HIP(hipGetLastError());
+            ++launches;
+            n_done = st + 1;
+        }
+    }
+
+    void state(std::vector<double> &out) override
+    {
+        std::vector<float> hm((size_t)rows * ld);
+        MP_HIP(hipMemcpy(hm.data(), d_m.p, hm.size() * sizeof(float), hipMemcpyDeviceToHost));
+        out.resize((size_t)reps * n * 3);
+        for (int rr = 0; rr < reps; ++rr)
+            for (int c = 0; c < 3 * n; ++c)
+                out[(size_t)rr * n * 3 + c] = hm[(size_t)rr * ld + c];
+    }
+
+    GroupOut collect(hipStream_t s) override
+    {
+        GroupOut o;
+        o.particle_of = slot_particle;
+        if (want_ens) {
+            hipLaunchKernelGGL(mp::ens_reduce_kernel, dim3((unsigned)(S * 3)), dim3(mp::kBlock), 0,
+                               s, d_partial.p, n_waves, S * 3, d_ens.p);
+            MP_HIP(hipGetLastError());
+        }
+        MP_HIP(hipStreamSynchronize(s));
+        state(o.state);
+        o.status.resize((size_t)reps * n);
+        MP_HIP(hipMemcpy(o.status.data(), d_status.p, o.status.size() * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+        if (want_traj) {
+            std::vector<float> hs(d_samples.n);
+            if (!hs.empty())
+                MP_HIP(hipMemcpy(hs.data(), d_samples.p, hs.size() * sizeof(float),
+                                 hipMemcpyDeviceToHost));
+            o.samples.assign(hs.begin(), hs.end());
+        }
+        if (want_ens) {
+            o.ens.resize((size_t)S * 3);
+            MP_HIP(hipMemcpy(o.ens.data(), d_ens.p, o.ens.size() * sizeof(double),
+                             hipMemcpyDeviceToHost));
+        }
+        return o;
+    }
+};
+
+// Every member is the same cluster: only seeds and initial states differ.
+bool shared_geometry(const simulation::EnsembleSpec &sp)
+{
+    const int64_t n = sp.member_offsets[1] - sp.member_offsets[0];
+    for (int64_t m = 1; m < sp.n_members; ++m) {
+        const int64_t b = sp.member_offsets[m];
+        if (sp.member_offsets[m + 1] - b != n) return false;
+        if (sp.magnetisation[m] != sp.magnetisation[0] || sp.damping[m] != sp.damping[0] ||
+            sp.temperature[m] != sp.temperature[0] || sp.field_shape[m] != sp.field_shape[0] ||
+            sp.field_amplitude[m] != sp.field_amplitude[0] ||
+            sp.field_frequency[m] != sp.field_frequency[0])
+            return false;
+        for (int64_t p = 0; p < n; ++p) {
+            if (sp.radius[b + p] != sp.radius[p] || sp.anisotropy[b + p] != sp.anisotropy[p])
+                return false;
+            for (int c = 0; c < 3; ++c)
+                if (sp.location[3 * (b + p) + c] != sp.location[3 * p + c] ||
+                    sp.anisotropy_axis[3 * (b + p) + c] != sp.anisotropy_axis[3 * p + c])
+                    return false;
+        }
+    }
+    return true;
+}
+
 }  // namespace
 
 struct mp_session {
@@ -572,6 +848,7 @@ struct mp_session {
     bool want_traj = false, want_ens = false;
     int device = 0;
     int cluster_block = 0;
+    int dipolar = 0;
     double kernel_ms = 0.0;
 };
 
@@ -662,6 +939,9 @@ void validate(const EnsembleSpec &sp)
     require(sp.precision == MP_PRECISION_FP32 || sp.precision == MP_PRECISION_FP64,
             "unknown precision");
     require(sp.noise == MP_NOISE_PHILOX || sp.noise == MP_NOISE_ARRAY, "unknown noise source");
+    require(sp.dipolar >= MP_DIPOLAR_AUTO && sp.dipolar <= MP_DIPOLAR_GEMM_F32,
+            "unknown dipolar mode");
+    require(sp.start_step >= 0, "start_step must be >= 0");
     require(sp.noise != MP_NOISE_ARRAY || sp.noise_array != nullptr,
             "noise_array is required for MP_NOISE_ARRAY");
     for (int64_t m = 0; m < sp.n_members; ++m) {
@@ -766,7 +1046,9 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
     h.M = sp.n_members;
     h.P = P;
     h.S = sp.max_samples;
-    h.m0.assign(sp.magnetisation_direction, sp.magnetisation_direction + 3 * P);
+    const double *m_start = sp.start_state ? sp.start_state : sp.magnetisation_direction;
+    h.m0.assign(m_start, m_start + 3 * P);
+    h.start = sp.start_step;
     h.ms.assign(sp.magnetisation, sp.magnetisation + sp.n_members);
     h.shape.assign(sp.field_shape, sp.field_shape + sp.n_members);
 
@@ -826,6 +1108,28 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
                 cluster_block = autotune_cluster_block(sp, plan, nmax, ss->sg.s);
         }
     }
+    // replicas of one cluster: the dipolar field as one product per stage
+    int gemm = 0;
+    if (cluster_block != 0 && G == 1 && sp.scheme == MP_SCHEME_HEUN &&
+        sp.noise == MP_NOISE_PHILOX && sp.precision == MP_PRECISION_FP32 &&
+        sp.dipolar != MP_DIPOLAR_PAIRS && shared_geometry(sp)) {
+        const int64_t n = sp.member_offsets[1] - sp.member_offsets[0];
+        if (sp.dipolar == MP_DIPOLAR_GEMM_BF16X3 || sp.dipolar == MP_DIPOLAR_GEMM_F32)
+            gemm = sp.dipolar;
+        else if (sp.n_members >= 64 && n >= 32)
+            gemm = MP_DIPOLAR_GEMM_BF16X3;
+    }
+    if (gemm != 0) {
+        ss->cluster_block = 0;
+        ss->dipolar = gemm;
+        ss->groups.emplace_back(new ReplicaRun(sp, plan, traj, ens, gemm));
+        return ss.release();
+    }
+    if (sp.dipolar == MP_DIPOLAR_GEMM_BF16X3 || sp.dipolar == MP_DIPOLAR_GEMM_F32)
+        throw simulation::Error(MP_ERR_UNSUPPORTED,
+                                "the dipolar product needs an FP32 Heun ensemble of replicas of "
+                                "one cluster with the Philox stream");
+    ss->dipolar = cluster_block ? MP_DIPOLAR_PAIRS : 0;
     ss->cluster_block = cluster_block;
     for (int g = 0; g < G; ++g) {
         if (sp.precision == MP_PRECISION_FP32 && sp.scheme == MP_SCHEME_HEUN)
@@ -875,7 +1179,9 @@ void session_finish(mp_session *ss, mp_ensemble_out &out)
         }
         newton_iters += o.newton_iters;
         newton_fail += o.newton_fail;
-        // samples recorded at step 0 are the initial state, exactly as given
+        // samples recorded at the start step are the start state exactly as
+        // given; samples before a resumed run's start step are not known
+        const double nan = std::numeric_limits<double>::quiet_NaN();
         if (want_traj)
             for (int64_t s = 0; s < S; ++s)
                 for (int i = 0; i < n; ++i) {
@@ -883,7 +1189,8 @@ void session_finish(mp_session *ss, mp_ensemble_out &out)
                     if (p < 0) continue;
                     for (int d = 0; d < 3; ++d)
                         traj[((size_t)s * 3 + d) * (size_t)P + (size_t)p] =
-                            sched[(size_t)s] == 0
+                            sched[(size_t)s] < h.start ? nan
+                            : sched[(size_t)s] == h.start
                                 ? h.m0[(size_t)p * 3 + d]
                                 : o.samples[((size_t)s * 3 + d) * (size_t)n + (size_t)i];
                 }
@@ -891,7 +1198,9 @@ void session_finish(mp_session *ss, mp_ensemble_out &out)
             for (int64_t s = 0; s < S; ++s)
                 for (int d = 0; d < 3; ++d) {
                     double v = o.ens[(size_t)s * 3 + d];
-                    if (sched[(size_t)s] == 0) {
+                    if (sched[(size_t)s] < h.start) {
+                        v = nan;
+                    } else if (sched[(size_t)s] == h.start) {
                         v = 0.0;
                         for (int i = 0; i < n; ++i) {
                             const int64_t p = o.particle_of[(size_t)i];
@@ -1051,7 +1360,7 @@ int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done
 }
 
 int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                    double *kernel_ms, int32_t *cluster_block)
+                    double *kernel_ms, int32_t *cluster_block, int32_t *dipolar)
 {
     return guarded([&] {
         if (!session) throw simulation::Error(MP_ERR_ARG, "session is required");
@@ -1064,6 +1373,7 @@ int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launch
         if (launches) *launches = l;
         if (kernel_ms) *kernel_ms = session->kernel_ms;
         if (cluster_block) *cluster_block = session->cluster_block;
+        if (dipolar) *dipolar = session->dipolar;
     });
 }
 
@@ -1106,6 +1416,68 @@ int mp_sample_schedule(double dt_reduced, double end_reduced, int64_t max_sample
         std::vector<int64_t> v;
         *n_steps = simulation::sample_schedule(dt_reduced, end_reduced, max_samples, v);
         std::memcpy(k, v.data(), v.size() * sizeof(int64_t));
+    });
+}
+
+int mp_dipolar_field(const double *location, const double *radius, const double *anisotropy,
+                     size_t n_particles, double magnetisation, const double *m, size_t reps,
+                     int mode, double *h, int device)
+{
+    return guarded([&] {
+        if (!location || !radius || !anisotropy || !m || !h || n_particles == 0 || reps == 0 ||
+            !(magnetisation > 0))
+            throw simulation::Error(MP_ERR_ARG, "invalid dipolar field arguments");
+        if (mode != MP_DIPOLAR_PAIRS && mode != MP_DIPOLAR_GEMM_BF16X3 && mode != MP_DIPOLAR_GEMM_F32)
+            throw simulation::Error(MP_ERR_ARG, "unknown dipolar mode");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw simulation::Error(MP_ERR_NODEV, "no HIP device visible");
+        if (device < 0 || device >= ndev) throw simulation::Error(MP_ERR_ARG, "bad device");
+        MP_HIP(hipSetDevice(device));
+        const simulation::ReducedMember r = simulation::reduce_member(
+            radius, anisotropy, n_particles, magnetisation, 0.1, 0.0, 1.0, 1.0, 0.0, 0.0);
+        const int ld = pad_to(3 * (int64_t)n_particles, mp::kGemmTile);
+        const int rows = pad_to((int64_t)reps, mp::kGemmTile);
+        std::vector<double> D;
+        dipole_tensor(r, location, magnetisation, n_particles, ld, D);
+        std::vector<double> ma((size_t)rows * ld, 0.0);
+        for (size_t rr = 0; rr < reps; ++rr)
+            for (size_t c = 0; c < 3 * n_particles; ++c) ma[rr * ld + c] = m[rr * 3 * n_particles + c];
+        std::vector<double> out((size_t)rows * ld);
+        if (mode == MP_DIPOLAR_PAIRS) {
+            DevBuf<double> da, db, dc((size_t)rows * ld);
+            da.upload(ma);
+            db.upload(D);
+            const int tot = rows * ld;
+            hipLaunchKernelGGL(mp::gemm_f64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                               0, 0, da.p, db.p, dc.p, rows, ld, ld);
+            MP_HIP(hipGetLastError());
+            MP_HIP(hipMemcpy(out.data(), dc.p, out.size() * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            DevBuf<float> a32, b32, c32((size_t)rows * ld);
+            DevBuf<__bf16> ahi, alo, bhi, blo;
+            a32.upload(cast_vec<float>(ma));
+            if (mode == MP_DIPOLAR_GEMM_F32) {
+                b32.upload(cast_vec<float>(D));
+            } else {
+                std::vector<__bf16> hi, lo;
+                split_host(D, hi, lo);
+                bhi.upload(hi);
+                blo.upload(lo);
+                ahi.alloc(ma.size());
+                alo.alloc(ma.size());
+                hipLaunchKernelGGL(mp::split_kernel, dim3((unsigned)((ma.size() + 255) / 256)),
+                                   dim3(256), 0, 0, a32.p, ahi.p, alo.p, (int64_t)ma.size());
+                MP_HIP(hipGetLastError());
+            }
+            launch_product(mode, a32.p, ahi.p, alo.p, b32.p, bhi.p, blo.p, c32.p, rows, ld, 0);
+            MP_HIP(hipGetLastError());
+            std::vector<float> o32(out.size());
+            MP_HIP(hipMemcpy(o32.data(), c32.p, o32.size() * sizeof(float), hipMemcpyDeviceToHost));
+            out.assign(o32.begin(), o32.end());
+        }
+        for (size_t rr = 0; rr < reps; ++rr)
+            for (size_t c = 0; c < 3 * n_particles; ++c) h[rr * 3 * n_particles + c] = out[rr * ld + c];
     });
 }
 

@@ -68,3 +68,8 @@ def test_errors_are_reported_not_raised_through_the_abi():
         magpy_amd.core.simulate_ensemble([0, 1], [1e-8], [2.3e4], [[0, 0, 1]], [[0, 0, 1]],
                                          [[0, 0, 0]], 4e5, 0.1, 300, 'constant', 0.0, 0.0, [1],
                                          1e-14, 1e-12, -1)
+    # a resumed run cannot start before step 0
+    with pytest.raises(ValueError):
+        magpy_amd.core.simulate_ensemble([0, 1], [1e-8], [2.3e4], [[0, 0, 1]], [[0, 0, 1]],
+                                         [[0, 0, 0]], 4e5, 0.1, 300, 'constant', 0.0, 0.0, [1],
+                                         1e-14, 1e-12, 4, start_step=-5)
