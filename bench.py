@@ -192,26 +192,43 @@ def cpu_baseline(name, inp, seconds=12.0):
                                                                      n_threads, m1)}
 
 
-def sar_run(n, precision, save):
+def sar_run(n, precision, save, stop=0, checkpoint="", resume=""):
     """The full AC-field workload (5 cycles) in one precision; prints progress
-    and the final-cycle hysteresis energy through EnsembleResults."""
+    and the final-cycle hysteresis energy through EnsembleResults.
+
+    A run longer than one sitting is split: ``stop`` ends it at that step and
+    writes the state to ``checkpoint`` (npz: step, state); ``resume`` starts
+    from such a file.  The Philox stream is indexed by step, so the split run
+    equals the unsplit one bit for bit."""
     import magpy_amd
     from magpy_amd.results import EnsembleResults
     core = magpy_amd.core
     inp = ac_field_inputs(n)
     f, cycles = 3e5, 5
     print("device:", core.device_info(0), flush=True)
+    start = {}
+    if resume:
+        ck = np.load(resume)
+        start = dict(start_step=int(ck["step"]), start_state=ck["state"])
+        print(json.dumps({"resume_from_step": start["start_step"]}), flush=True)
     ses = core.Session(precision=precision, trajectories=False, ensemble_sum=True,
                        time_step=1e-14, end_time=cycles / f, max_samples=2000 * cycles + 1,
-                       interactions=False, use_implicit=False, **inp)
+                       interactions=False, use_implicit=False, **start, **inp)
     total = ses.info()["n_steps"]
+    upto = min(stop, total) if stop else total
     t0 = time.perf_counter()
-    done = 0
+    done = start.get("start_step", 0)
     chunk = 1 << 24
-    while done < total:
-        done = ses.advance(chunk)
+    while done < upto:
+        done = ses.advance(min(chunk, upto - done))
         print(json.dumps({"progress_steps": int(done), "of": int(total),
                           "elapsed_s": round(time.perf_counter() - t0, 1)}), flush=True)
+    if done < total:
+        state, step = ses.state()
+        np.savez(checkpoint, step=step, state=state)
+        print(json.dumps({"checkpoint": checkpoint, "step": int(step),
+                          "wall_s": time.perf_counter() - t0}), flush=True)
+        return
     res = ses.finish()
     el = time.perf_counter() - t0
     er = EnsembleResults.from_batch(res, inp["member_offsets"])
@@ -259,12 +276,17 @@ def main():
     ap.add_argument("--sar-run", default="", choices=["", "fp32", "fp64"],
                     help="run the whole AC-field workload in this precision and report SAR")
     ap.add_argument("--save", default="", help="npz file for the --sar-run time series")
+    ap.add_argument("--sar-stop", type=int, default=0,
+                    help="end the --sar-run at this step and write --sar-checkpoint")
+    ap.add_argument("--sar-checkpoint", default="", help="npz written at --sar-stop")
+    ap.add_argument("--sar-resume", default="", help="npz checkpoint to resume --sar-run from")
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
         return
     if args.sar_run:
-        sar_run(args.particles or 65536, args.sar_run, args.save)
+        sar_run(args.particles or 65536, args.sar_run, args.save, args.sar_stop,
+                args.sar_checkpoint, args.sar_resume)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build's path runs on a single MI355X "

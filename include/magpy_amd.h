@@ -195,6 +195,70 @@ int mp_dipolar_field(const double *location, const double *radius, const double 
 int mp_wiener_normals(uint64_t seed, uint32_t particle, uint32_t tag, uint64_t j0, size_t n,
                       int precision, double *out, int device);
 
+/* ------------------------------------------------------------ sensitivities
+ * Pathwise gradients of an ensemble of independent particles (one particle
+ * per member, as EnsembleModel builds from one Model, magpy/model.py:146-208)
+ * with respect to theta = (anisotropy K [J/m^3], damping alpha, field
+ * amplitude H [A/m], median radius r [m]).  Particle p has radius
+ * r * radius_shape[p] and its own seed; Ms, T, the sine field's frequency
+ * and the time step are fixed.  The step is integrator::heun
+ * (lib/integrators.cpp:372-405) in the reduced units of
+ * lib/simulation.cpp:501-549, with the field's phase sin(2 pi f dt k) taken
+ * at step k in SI units; the Wiener increments are the MP_NOISE_PHILOX
+ * stream of (seed[p], particle 0, noise_tag) and do not depend on theta.
+ * Outputs of a forward pass: ens_mz[s] = Ms * mean_p m_z,p after
+ * sample_steps[s] steps (the reference's ensemble_magnetisation('z'),
+ * magpy/results.py:134-151, at those steps) and final_mz = mean_p m_z,p
+ * after n_steps.  A reverse pass returns the gradient of
+ *   J = sum_s grad_ens_mz[s] * ens_mz[s] + grad_final_mz * final_mz
+ * with respect to theta through the trajectories (the vector-Jacobian
+ * product): the forward states are kept every checkpoint_interval steps in
+ * device memory and recomputed in between.  No counterpart in the reference,
+ * which has no derivatives; magpy_amd.adjoint wraps these calls in a
+ * torch.autograd.Function. */
+typedef struct mp_adjoint_spec {
+    int64_t n_particles;
+    const double *radius_shape;             /* [P] radius / median radius */
+    const double *anisotropy_axis;          /* [P][3] */
+    const double *magnetisation_direction;  /* [P][3] initial state */
+    const int64_t *seed;                    /* [P] */
+    double magnetisation;                   /* Ms [A/m] */
+    double temperature;                     /* [K] */
+    double field_frequency;                 /* [Hz] */
+    double time_step;                       /* [s] */
+    int64_t n_steps;
+    const int64_t *sample_steps;            /* [S] ascending, in [0, n_steps] */
+    int64_t n_samples;
+    int64_t checkpoint_interval;            /* steps; a positive multiple of 4 */
+    int32_t precision;                      /* MP_PRECISION_*: state and adjoint */
+    uint32_t noise_tag;
+    int32_t device;
+} mp_adjoint_spec;
+
+typedef struct mp_adjoint mp_adjoint;
+int mp_adjoint_create(const mp_adjoint_spec *spec, mp_adjoint **out);
+void mp_adjoint_destroy(mp_adjoint *a);
+/* start a forward pass at theta (drops any earlier pass) */
+int mp_adjoint_begin(mp_adjoint *a, const double theta[4]);
+/* advance the forward pass by up to max_steps; *steps_done = steps so far */
+int mp_adjoint_forward_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_done);
+/* finish the forward pass and return its outputs (ens_mz [S], final_mz);
+ * *status (optional) = OR over particles of MP_STATUS_NONFINITE */
+int mp_adjoint_forward_result(mp_adjoint *a, double *ens_mz, double *final_mz, uint32_t *status);
+/* start a reverse pass of the finished forward pass for the given weights */
+int mp_adjoint_reverse_begin(mp_adjoint *a, const double *grad_ens_mz, double grad_final_mz);
+/* advance the reverse pass by up to max_steps; *steps_left = steps still to go */
+int mp_adjoint_reverse_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_left);
+/* finish the reverse pass: grad_theta[4] = dJ/d(K, alpha, H, r) */
+int mp_adjoint_reverse_result(mp_adjoint *a, double grad_theta[4]);
+/* device milliseconds of the forward and reverse kernels so far and the
+ * device bytes held for checkpoints and segment scratch */
+int mp_adjoint_info(mp_adjoint *a, double *forward_ms, double *reverse_ms, int64_t *checkpoint_bytes,
+                    int64_t *scratch_bytes);
+/* Reduced per-particle parameters q[p] = (dt, sr sqrt(dt), alpha, happ) at
+ * theta and their derivatives dq[p][j][i] = dq_j / dtheta_i (host only) */
+int mp_adjoint_reduced(const mp_adjoint_spec *spec, const double theta[4], double *q, double *dq);
+
 /* get_KB / get_mu0 / get_gamma (magpy/core.pyx:29-34) */
 void mp_constants(double *kb, double *mu0, double *gamma);
 

@@ -2,9 +2,9 @@
 
   python -m magpy_amd.build        # or magpy_amd.build.build()
 
-1. libmagpy_amd.so   hipcc, gfx950, from csrc/ensemble_run.hip (kernels in
-                     csrc/sllg_kernels.hpp); exports the C ABI of
-                     include/magpy_amd.h.
+1. libmagpy_amd.so   hipcc, gfx950, from csrc/ensemble_run.hip and
+                     csrc/adjoint.hip (kernels in csrc/*_kernels.hpp);
+                     exports the C ABI of include/magpy_amd.h.
 2. core.*.so         Cython binding core.pyx, linked against libmagpy_amd.so
                      with an $ORIGIN rpath.
 
@@ -39,11 +39,14 @@ def core_ext_path():
     return os.path.join(HERE, "core" + suffix)
 
 
+SOURCES = ["ensemble_run.hip", "adjoint.hip"]
+
+
 def build_lib():
-    src = os.path.join(HERE, "csrc", "ensemble_run.hip")
+    srcs = [os.path.join(HERE, "csrc", s) for s in SOURCES]
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
-    _run([hipcc] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include"), src, "-o", tmp])
+    _run([hipcc] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include")] + srcs + ["-o", tmp])
     os.replace(tmp, LIB)
 
 

@@ -79,6 +79,38 @@ cdef extern from "magpy_amd.h" nogil:
                            int64_t *k, int64_t *n_steps)
     int mp_wiener_normals(uint64_t seed, uint32_t particle, uint32_t tag, uint64_t j0, size_t n,
                           int precision, double *out, int device)
+    ctypedef struct mp_adjoint_spec:
+        int64_t n_particles
+        const double *radius_shape
+        const double *anisotropy_axis
+        const double *magnetisation_direction
+        const int64_t *seed
+        double magnetisation
+        double temperature
+        double field_frequency
+        double time_step
+        int64_t n_steps
+        const int64_t *sample_steps
+        int64_t n_samples
+        int64_t checkpoint_interval
+        int32_t precision
+        uint32_t noise_tag
+        int32_t device
+    ctypedef struct mp_adjoint:
+        pass
+    int mp_adjoint_create(const mp_adjoint_spec *spec, mp_adjoint **out)
+    void mp_adjoint_destroy(mp_adjoint *a)
+    int mp_adjoint_begin(mp_adjoint *a, const double *theta)
+    int mp_adjoint_forward_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_done)
+    int mp_adjoint_forward_result(mp_adjoint *a, double *ens_mz, double *final_mz,
+                                  uint32_t *status)
+    int mp_adjoint_reverse_begin(mp_adjoint *a, const double *grad_ens_mz, double grad_final_mz)
+    int mp_adjoint_reverse_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_left)
+    int mp_adjoint_reverse_result(mp_adjoint *a, double *grad_theta)
+    int mp_adjoint_info(mp_adjoint *a, double *forward_ms, double *reverse_ms,
+                        int64_t *checkpoint_bytes, int64_t *scratch_bytes)
+    int mp_adjoint_reduced(const mp_adjoint_spec *spec, const double *theta, double *q,
+                           double *dq)
     void mp_constants(double *kb, double *mu0, double *gamma)
     const char *mp_last_error()
     const char *mp_version()
@@ -483,3 +515,171 @@ cpdef simulate(radius, anisotropy, anisotropy_axis, magnetisation_direction, loc
         'y': {i: res['y'][i] for i in range(n)},
         'z': {i: res['z'][i] for i in range(n)},
     }
+
+
+# ------------------------------------------------------------------ sensitivities
+cdef class _AdjointSpec:
+    """mp_adjoint_spec with the arrays it points to."""
+    cdef mp_adjoint_spec sp
+    cdef object arrays
+
+    def __init__(self, radius_shape, anisotropy_axis, magnetisation_direction, seeds,
+                 double magnetisation, double temperature, double field_frequency,
+                 double time_step, int64_t n_steps, sample_steps, int64_t checkpoint_interval=1000,
+                 precision='fp32', uint32_t noise_tag=0, int device=0):
+        shp = _c(radius_shape, np.float64).reshape(-1)
+        P = shp.shape[0]
+        if P == 0:
+            raise ValueError("no particles")
+        ax = _c(np.reshape(anisotropy_axis, (P, 3)), np.float64)
+        m0 = _c(np.reshape(magnetisation_direction, (P, 3)), np.float64)
+        sd = _c(seeds, np.int64).reshape(-1)
+        ks = _c(sample_steps, np.int64).reshape(-1)
+        if sd.shape[0] != P:
+            raise ValueError("one seed per particle")
+        self.arrays = (shp, ax, m0, sd, ks)
+        cdef const double[::1] shp_v = shp
+        cdef const double[:, ::1] ax_v = ax, m0_v = m0
+        cdef const int64_t[::1] sd_v = sd
+        cdef const int64_t[::1] ks_v
+        self.sp.n_particles = P
+        self.sp.radius_shape = &shp_v[0]
+        self.sp.anisotropy_axis = &ax_v[0, 0]
+        self.sp.magnetisation_direction = &m0_v[0, 0]
+        self.sp.seed = &sd_v[0]
+        self.sp.magnetisation = magnetisation
+        self.sp.temperature = temperature
+        self.sp.field_frequency = field_frequency
+        self.sp.time_step = time_step
+        self.sp.n_steps = n_steps
+        self.sp.n_samples = ks.shape[0]
+        if ks.shape[0]:
+            ks_v = ks
+            self.sp.sample_steps = &ks_v[0]
+        else:
+            self.sp.sample_steps = NULL
+        self.sp.checkpoint_interval = checkpoint_interval
+        self.sp.precision = PRECISIONS[precision]
+        self.sp.noise_tag = noise_tag
+        self.sp.device = device
+
+
+def adjoint_reduced(theta, radius_shape, double magnetisation, double temperature,
+                    double time_step):
+    """Reduced parameters q [P, 4] = (dt, sr sqrt(dt), alpha, happ) of every
+    particle at theta = (K, alpha, H, median radius) and their derivatives
+    dq [P, 4, 4] (dq[p, j, i] = dq_j / dtheta_i); host-side, no device."""
+    shp = _c(radius_shape, np.float64).reshape(-1)
+    P = shp.shape[0]
+    cdef _AdjointSpec spec = _AdjointSpec(shp, np.zeros((P, 3)), np.zeros((P, 3)),
+                                          np.zeros(P, np.int64), magnetisation, temperature,
+                                          0.0, time_step, 1, [])
+    th = _c(theta, np.float64)
+    q = np.zeros((P, 4))
+    dq = np.zeros((P, 4, 4))
+    cdef const double[::1] th_v = th
+    cdef double[:, ::1] q_v = q
+    cdef double[:, :, ::1] dq_v = dq
+    _check(mp_adjoint_reduced(&spec.sp, &th_v[0], &q_v[0, 0], &dq_v[0, 0, 0]))
+    return q, dq
+
+
+cdef class Adjoint:
+    """Forward and reverse passes of the pathwise sensitivities
+    (mp_adjoint_*).  theta = (anisotropy K, damping alpha, field amplitude H,
+    median radius).  forward(theta) -> (ens_mz [S], final_mz, status);
+    reverse(grad_ens_mz, grad_final_mz) -> dJ/dtheta [4] for the last
+    forward pass.  Both also exist as begin/advance/result steps."""
+    cdef mp_adjoint *a
+    cdef _AdjointSpec spec
+    cdef readonly int64_t n_particles, n_steps, n_samples
+    cdef readonly int64_t forward_passes
+
+    def __cinit__(self):
+        self.a = NULL
+
+    def __init__(self, *args, **kw):
+        self.spec = _AdjointSpec(*args, **kw)
+        cdef mp_adjoint *ptr = NULL
+        cdef int rc
+        with nogil:
+            rc = mp_adjoint_create(&self.spec.sp, &ptr)
+        _check(rc)
+        self.a = ptr
+        self.n_particles = self.spec.sp.n_particles
+        self.n_steps = self.spec.sp.n_steps
+        self.n_samples = self.spec.sp.n_samples
+        self.forward_passes = 0
+
+    def __dealloc__(self):
+        if self.a != NULL:
+            mp_adjoint_destroy(self.a)
+            self.a = NULL
+
+    def begin(self, theta):
+        th = _c(theta, np.float64)
+        if th.shape[0] != 4:
+            raise ValueError("theta has four entries (K, alpha, H, radius)")
+        cdef const double[::1] th_v = th
+        _check(mp_adjoint_begin(self.a, &th_v[0]))
+        self.forward_passes += 1
+
+    def forward_advance(self, int64_t n):
+        cdef int64_t done = 0
+        cdef int rc
+        with nogil:
+            rc = mp_adjoint_forward_advance(self.a, n, &done)
+        _check(rc)
+        return done
+
+    def forward_result(self):
+        ens = np.zeros(max(self.n_samples, 1))
+        cdef double[::1] ev = ens
+        cdef double fin = 0
+        cdef uint32_t st = 0
+        cdef int rc
+        with nogil:
+            rc = mp_adjoint_forward_result(self.a, &ev[0], &fin, &st)
+        _check(rc)
+        return ens[:self.n_samples], fin, st
+
+    def forward(self, theta):
+        self.begin(theta)
+        return self.forward_result()
+
+    def reverse_begin(self, grad_ens_mz, double grad_final_mz):
+        g = _c(grad_ens_mz, np.float64).reshape(-1)
+        if g.shape[0] != self.n_samples:
+            raise ValueError("grad_ens_mz needs one entry per sample")
+        if g.shape[0] == 0:
+            g = np.zeros(1)
+        cdef const double[::1] gv = g
+        _check(mp_adjoint_reverse_begin(self.a, &gv[0], grad_final_mz))
+
+    def reverse_advance(self, int64_t n):
+        cdef int64_t left = 0
+        cdef int rc
+        with nogil:
+            rc = mp_adjoint_reverse_advance(self.a, n, &left)
+        _check(rc)
+        return left
+
+    def reverse_result(self):
+        grad = np.zeros(4)
+        cdef double[::1] gv = grad
+        cdef int rc
+        with nogil:
+            rc = mp_adjoint_reverse_result(self.a, &gv[0])
+        _check(rc)
+        return grad
+
+    def reverse(self, grad_ens_mz, double grad_final_mz):
+        self.reverse_begin(grad_ens_mz, grad_final_mz)
+        return self.reverse_result()
+
+    def info(self):
+        cdef double fms = 0, rms = 0
+        cdef int64_t ckb = 0, scb = 0
+        _check(mp_adjoint_info(self.a, &fms, &rms, &ckb, &scb))
+        return {'forward_ms': fms, 'reverse_ms': rms, 'checkpoint_bytes': ckb,
+                'scratch_bytes': scb}

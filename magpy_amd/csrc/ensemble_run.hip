@@ -22,87 +22,22 @@
 #include <tuple>
 #include <vector>
 
+#include "host_util.hpp"
 #include "simulation.hpp"
 #include "replica_kernels.hpp"
 #include "sllg_kernels.hpp"
 
+namespace mp_host {
+std::string &last_error()
+{
+    thread_local std::string text;
+    return text;
+}
+}  // namespace mp_host
+
 namespace {
 
-// include/constants.hpp:10-12
-constexpr double kKB = 1.38064852e-23;
-constexpr double kMU0 = 1.25663706e-6;
-constexpr double kGYROMAG = 1.76086e11;
-constexpr double kPiHost = 3.14159265358979323846;
-
-thread_local std::string g_last_error;
-
-#define MP_HIP(call)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            throw simulation::Error(MP_ERR_HIP, std::string(#call) + ": " +                 \
-                                                    hipGetErrorString(e_));                 \
-    } while (0)
-
-void require(bool ok, const std::string &msg)
-{
-    if (!ok) throw simulation::Error(MP_ERR_ARG, msg);
-}
-
-// Device allocation owned by its holder.
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    explicit DevBuf(size_t count) { alloc(count); }
-    void alloc(size_t count)
-    {
-        release();
-        n = count;
-        if (count) {
-            hipError_t e = hipMalloc(&p, count * sizeof(T));
-            if (e != hipSuccess) {
-                p = nullptr;
-                n = 0;
-                throw simulation::Error(MP_ERR_ALLOC, "hipMalloc of " +
-                                                          std::to_string(count * sizeof(T)) +
-                                                          " bytes failed");
-            }
-        }
-    }
-    void upload(const std::vector<T> &h)
-    {
-        alloc(h.size());
-        if (!h.empty()) MP_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { release(); }
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-};
-
-struct StreamGuard {
-    hipStream_t s = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    StreamGuard()
-    {
-        MP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        MP_HIP(hipEventCreate(&e0));
-        MP_HIP(hipEventCreate(&e1));
-    }
-    ~StreamGuard()
-    {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        if (s) (void)hipStreamDestroy(s);
-    }
-};
+using namespace mp_host;
 
 // The applied field of lib/simulation.cpp:552-573 at reduced time t.
 double applied_host(int shape, double happ, double fapp, double t)
@@ -1248,29 +1183,6 @@ void session_finish(mp_session *ss, mp_ensemble_out &out)
     out.kernel_ms = ss->kernel_ms;
 }
 
-int fail(const simulation::Error &e)
-{
-    g_last_error = e.what();
-    return e.code;
-}
-
-template <typename F>
-int guarded(F &&f)
-{
-    try {
-        f();
-        return MP_OK;
-    } catch (const simulation::Error &e) {
-        return fail(e);
-    } catch (const std::bad_alloc &) {
-        g_last_error = "host allocation failed";
-        return MP_ERR_ALLOC;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return MP_ERR_ARG;
-    }
-}
-
 }  // namespace
 
 namespace simulation {
@@ -1514,7 +1426,7 @@ void mp_constants(double *kb, double *mu0, double *gamma)
     if (gamma) *gamma = kGYROMAG;
 }
 
-const char *mp_last_error(void) { return g_last_error.c_str(); }
+const char *mp_last_error(void) { return last_error().c_str(); }
 
 const char *mp_version(void) { return "magpy_amd 0.1 (abi 1, gfx950)"; }
 
@@ -1523,7 +1435,7 @@ int mp_device_info(int device, char *buf, size_t len)
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) {
-        g_last_error = hipGetErrorString(e);
+        last_error() = hipGetErrorString(e);
         return MP_ERR_NODEV;
     }
     if (buf && len)

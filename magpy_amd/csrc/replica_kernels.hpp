@@ -51,7 +51,7 @@ __device__ inline void store_tile32(float *C, int ldc, int r0, int c0, const f32
 // Ht = Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T.  A = Mt [rows][K],
 // B = D [cols][K], both row-major BF16; C [rows][ldc] FP32.  rows, cols are
 // multiples of kGemmTile and K of 16 (the host pads with zeros).
-__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const __bf16 *__restrict__ a_hi,
+static __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const __bf16 *__restrict__ a_hi,
                                                           const __bf16 *__restrict__ a_lo,
                                                           const __bf16 *__restrict__ b_hi,
                                                           const __bf16 *__restrict__ b_lo,
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const __bf16 *__restri
 // The same product in FP32 on MFMA (v_mfma_f32_32x32x2f32).  Each lane reads
 // four consecutive k of its rows per step of 8 and feeds them to four
 // instructions; A and B use the same k assignment, so the sum is complete.
-__global__ __launch_bounds__(256) void gemm_f32_kernel(const float *__restrict__ a,
+static __global__ __launch_bounds__(256) void gemm_f32_kernel(const float *__restrict__ a,
                                                        const float *__restrict__ b,
                                                        float *__restrict__ c, int K, int ldc)
 {
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float *__restrict__
 }
 
 // FP64 product on the VALU, one output per lane (reference for tests).
-__global__ void gemm_f64_kernel(const double *__restrict__ a, const double *__restrict__ b,
+static __global__ void gemm_f64_kernel(const double *__restrict__ a, const double *__restrict__ b,
                                 double *__restrict__ c, int rows, int cols, int K)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -162,7 +162,7 @@ __global__ void gemm_f64_kernel(const double *__restrict__ a, const double *__re
 }
 
 // FP32 -> (hi, lo) BF16 split of n values.
-__global__ void split_kernel(const float *__restrict__ x, __bf16 *__restrict__ hi,
+static __global__ void split_kernel(const float *__restrict__ x, __bf16 *__restrict__ hi,
                              __bf16 *__restrict__ lo, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,7 +225,7 @@ __device__ inline void replica_normals(const ReplicaArgs &a, int r, int p, int64
 }
 
 // stage 1: m~ = m + L(m, h dt + s), stores m~, the increment and the split of m~
-__global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs a, int64_t n_step, float h1)
+static __global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs a, int64_t n_step, float h1)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n * a.reps) return;
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs a, int6
 }
 
 // stage 2: m' = m + (L(m, v1) + L(m~, v2)) / 2, stores m', its split, samples
-__global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs a, int64_t n_step, float h2,
+static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs a, int64_t n_step, float h2,
                                                              int64_t s_first, int n_rec)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a,
 
 // Applied field of a uniform launch: out[i] = h(t_{base+1+i}), i < count,
 // evaluated in FP64 from the cycle count c = f_red t reduced to [0,1).
-__global__ void field_table_kernel(int shape, double amp, double fdt, int64_t base, int64_t count,
+static __global__ void field_table_kernel(int shape, double amp, double fdt, int64_t base, int64_t count,
                                    float *out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(kBlock) void implicit_indep_kernel(ImplicitArgs ia)
 }
 
 // Sum per-wave partials of each (sample, component) in a fixed order.
-__global__ void ens_reduce_kernel(const double *partial, int n_waves, int64_t rows, double *out)
+static __global__ void ens_reduce_kernel(const double *partial, int n_waves, int64_t rows, double *out)
 {
     const int64_t r = blockIdx.x;
     if (r >= rows) return;

@@ -33,6 +33,14 @@ SDEJAC_FN = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_double, C
 FJ_FN = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp, _dp)
 
 
+class PathProblem(C.Structure):
+    """orc_path_problem"""
+    _fields_ = [("n_particles", C.c_size_t), ("axis", _dp), ("m0", _dp), ("dt", _dp),
+                ("sig", _dp), ("alpha", _dp), ("happ", _dp), ("fdt", C.c_double),
+                ("seed", C.POINTER(C.c_uint64)), ("tag", C.c_uint32), ("n_steps", C.c_int64),
+                ("sample_steps", _i64p), ("n_samples", C.c_size_t)]
+
+
 class Stats(C.Structure):
     _fields_ = [("n_steps", C.c_int64), ("newton_iters", C.c_int64),
                 ("newton_failures", C.c_int64), ("rng_exhausted", C.c_int)]
@@ -101,6 +109,14 @@ def lib():
                                             _dp, _dp, SDEJAC_FN, C.c_void_p, C.c_size_t,
                                             C.c_size_t, C.c_double, C.c_double, C.c_double,
                                             C.c_size_t, _ip]
+        L.orc_path_phase.restype = C.c_double
+        L.orc_path_phase.argtypes = [C.c_double, C.c_int64]
+        L.orc_path_step.restype = None
+        L.orc_path_step.argtypes = [_dp, _dp, _dp] + [C.c_double] * 6
+        L.orc_path_run.restype = None
+        L.orc_path_run.argtypes = [C.POINTER(PathProblem), _dp, _dp, _dp]
+        L.orc_path_tangent.restype = None
+        L.orc_path_tangent.argtypes = [C.POINTER(PathProblem), C.c_size_t, _dp, _dp, _dp]
         L.orc_sample_schedule.restype = C.c_int64
         L.orc_sample_schedule.argtypes = [C.c_double, C.c_double, C.c_size_t, _i64p]
         L.orc_full_dynamics.restype = C.c_int
@@ -242,3 +258,42 @@ def reduced_parameters(radius, anisotropy, magnetisation, damping, temperature, 
     return dict(volume=vol, v_av=vav, k_av=kav, hk=hk, time_factor=tf, dt=time_step * tf,
                 end=end_time * tf, sr=sr, happ=field_amplitude / hk,
                 fapp=field_frequency / tf)
+
+
+# ---------------------------------------------------------------- pathwise sensitivities
+def path_problem(axis, m0, dt, sig, alpha, happ, fdt, seeds, n_steps, sample_steps, tag=0):
+    """orc_path_problem over P particles; dt/sig/alpha/happ are scalars or [P].
+    Returns (struct, keep-alive arrays)."""
+    axis = f64(np.reshape(axis, (-1, 3)))
+    P = axis.shape[0]
+    per = [f64(np.broadcast_to(np.asarray(x, dtype=np.float64), (P,))) for x in
+           (dt, sig, alpha, happ)]
+    m0 = f64(np.reshape(m0, (P, 3)))
+    sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+    ks = np.ascontiguousarray(np.asarray(sample_steps, dtype=np.int64))
+    pr = PathProblem(P, ptr(axis), ptr(m0), ptr(per[0]), ptr(per[1]), ptr(per[2]), ptr(per[3]),
+                     float(fdt), sd.ctypes.data_as(C.POINTER(C.c_uint64)), tag, int(n_steps),
+                     ptr(ks) if ks.size else None, ks.size)
+    return pr, (axis, m0, per, sd, ks)
+
+
+def path_run(pr):
+    """(mz_sum [S], mz_final_sum, final_state [P,3]) of orc_path_run."""
+    S, P = pr[0].n_samples, pr[0].n_particles
+    mz = np.zeros(S)
+    fin = C.c_double()
+    st = np.zeros((P, 3))
+    lib().orc_path_run(C.byref(pr[0]), ptr(mz) if S else None, C.byref(fin), ptr(st))
+    return mz, fin.value, st
+
+
+def path_tangent(pr, dq):
+    """dq [P, D, 4] (derivatives of dt, sig, alpha, happ along D directions)
+    -> (dmz_sum [S, D], dmz_final_sum [D])."""
+    dq = f64(dq)
+    P, D = dq.shape[0], dq.shape[1]
+    S = pr[0].n_samples
+    out = np.zeros((S, D))
+    fin = np.zeros(D)
+    lib().orc_path_tangent(C.byref(pr[0]), D, ptr(dq), ptr(out) if S else None, ptr(fin))
+    return out, fin

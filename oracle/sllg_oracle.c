@@ -919,3 +919,142 @@ int orc_full_dynamics(
     free(volume); free(v_red); free(k_red); free(sr); free(vecs); free(mags); free(units);
     return rc;
 }
+
+/* ======================================================================
+ * Pathwise sensitivities of the Heun path (see sllg_oracle.h).  The
+ * tangent differentiates each operation of the step directly (forward
+ * mode, product rule), independently of any transposed form.
+ * ==================================================================== */
+double orc_path_phase(double fdt, int64_t k)
+{
+    double c = (double)k * fdt;
+    c -= floor(c);
+    return sin(2.0 * M_PI * c);
+}
+
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* L(m, v) = -m x v + alpha (v |m|^2 - m (m.v)) */
+static void path_L(const double *m, const double *v, double alpha, double *out)
+{
+    double mm = dot3(m, m), mv = dot3(m, v);
+    out[0] = -(m[1] * v[2] - m[2] * v[1]) + alpha * (v[0] * mm - m[0] * mv);
+    out[1] = -(m[2] * v[0] - m[0] * v[2]) + alpha * (v[1] * mm - m[1] * mv);
+    out[2] = -(m[0] * v[1] - m[1] * v[0]) + alpha * (v[2] * mm - m[2] * mv);
+}
+
+/* directional derivative of L along (dm, dv, dalpha) */
+static void path_dL(const double *m, const double *v, double alpha, const double *dm,
+                    const double *dv, double dalpha, double *out)
+{
+    double mm = dot3(m, m), mv = dot3(m, v);
+    double dmm = 2.0 * dot3(m, dm), dmv = dot3(dm, v) + dot3(m, dv);
+    double dc[3];
+    /* d(-m x v) = -dm x v - m x dv */
+    dc[0] = -(dm[1] * v[2] - dm[2] * v[1]) - (m[1] * dv[2] - m[2] * dv[1]);
+    dc[1] = -(dm[2] * v[0] - dm[0] * v[2]) - (m[2] * dv[0] - m[0] * dv[2]);
+    dc[2] = -(dm[0] * v[1] - dm[1] * v[0]) - (m[0] * dv[1] - m[1] * dv[0]);
+    for (int i = 0; i < 3; ++i) {
+        double g = v[i] * mm - m[i] * mv;
+        double dg = dv[i] * mm + v[i] * dmm - dm[i] * mv - m[i] * dmv;
+        out[i] = dc[i] + dalpha * g + alpha * dg;
+    }
+}
+
+static void path_v(const double *m, const double *e, const double *w, double dt, double sig,
+                   double happ, double s, double *v)
+{
+    double me = dot3(m, e);
+    for (int i = 0; i < 3; ++i) v[i] = dt * (me * e[i] + (i == 2 ? happ * s : 0.0)) + sig * w[i];
+}
+
+static void path_dv(const double *m, const double *dm, const double *e, const double *w,
+                    double dt, double happ, double s, const double *dq, double *dv)
+{
+    double me = dot3(m, e), dme = dot3(dm, e);
+    for (int i = 0; i < 3; ++i) {
+        double h = me * e[i] + (i == 2 ? happ * s : 0.0);
+        double dh = dme * e[i] + (i == 2 ? dq[3] * s : 0.0);
+        dv[i] = dq[0] * h + dt * dh + dq[1] * w[i];
+    }
+}
+
+void orc_path_step(double m[3], const double e[3], const double w[3], double dt, double sig,
+                   double alpha, double happ, double s1, double s2)
+{
+    double v1[3], a1[3], mt[3], v2[3], a2[3];
+    path_v(m, e, w, dt, sig, happ, s1, v1);
+    path_L(m, v1, alpha, a1);
+    for (int i = 0; i < 3; ++i) mt[i] = m[i] + a1[i];
+    path_v(mt, e, w, dt, sig, happ, s2, v2);
+    path_L(mt, v2, alpha, a2);
+    for (int i = 0; i < 3; ++i) m[i] = m[i] + 0.5 * (a1[i] + a2[i]);
+}
+
+void orc_path_run(const orc_path_problem *pr, double *mz_sum, double *mz_final_sum,
+                  double *final_state)
+{
+    for (size_t s = 0; s < pr->n_samples; ++s) mz_sum[s] = 0.0;
+    *mz_final_sum = 0.0;
+    for (size_t p = 0; p < pr->n_particles; ++p) {
+        double m[3] = {pr->m0[3 * p], pr->m0[3 * p + 1], pr->m0[3 * p + 2]};
+        const double *e = pr->axis + 3 * p;
+        size_t s = 0;
+        for (int64_t n = 0;; ++n) {
+            while (s < pr->n_samples && pr->sample_steps[s] == n) mz_sum[s++] += m[2];
+            if (n == pr->n_steps) break;
+            double w[3];
+            orc_wiener_normals(pr->seed[p], 0, pr->tag, 3 * (uint64_t)n, 3, w);
+            orc_path_step(m, e, w, pr->dt[p], pr->sig[p], pr->alpha[p], pr->happ[p],
+                          orc_path_phase(pr->fdt, n + 1), orc_path_phase(pr->fdt, n + 2));
+        }
+        *mz_final_sum += m[2];
+        if (final_state)
+            for (int i = 0; i < 3; ++i) final_state[3 * p + i] = m[i];
+    }
+}
+
+void orc_path_tangent(const orc_path_problem *pr, size_t n_dirs, const double *dq,
+                      double *dmz_sum, double *dmz_final_sum)
+{
+    for (size_t i = 0; i < pr->n_samples * n_dirs; ++i) dmz_sum[i] = 0.0;
+    for (size_t d = 0; d < n_dirs; ++d) dmz_final_sum[d] = 0.0;
+    double *tm = (double *)calloc(3 * n_dirs, sizeof(double));
+    for (size_t p = 0; p < pr->n_particles; ++p) {
+        double m[3] = {pr->m0[3 * p], pr->m0[3 * p + 1], pr->m0[3 * p + 2]};
+        const double *e = pr->axis + 3 * p;
+        const double dt = pr->dt[p], sig = pr->sig[p], al = pr->alpha[p], ha = pr->happ[p];
+        memset(tm, 0, 3 * n_dirs * sizeof(double));
+        size_t s = 0;
+        for (int64_t n = 0;; ++n) {
+            while (s < pr->n_samples && pr->sample_steps[s] == n) {
+                for (size_t d = 0; d < n_dirs; ++d) dmz_sum[s * n_dirs + d] += tm[3 * d + 2];
+                ++s;
+            }
+            if (n == pr->n_steps) break;
+            double w[3];
+            orc_wiener_normals(pr->seed[p], 0, pr->tag, 3 * (uint64_t)n, 3, w);
+            const double s1 = orc_path_phase(pr->fdt, n + 1), s2 = orc_path_phase(pr->fdt, n + 2);
+            double v1[3], a1[3], mt[3], v2[3], a2[3];
+            path_v(m, e, w, dt, sig, ha, s1, v1);
+            path_L(m, v1, al, a1);
+            for (int i = 0; i < 3; ++i) mt[i] = m[i] + a1[i];
+            path_v(mt, e, w, dt, sig, ha, s2, v2);
+            path_L(mt, v2, al, a2);
+            for (size_t d = 0; d < n_dirs; ++d) {
+                const double *q = dq + (p * n_dirs + d) * 4;
+                double *dm = tm + 3 * d;
+                double dv1[3], da1[3], dmt[3], dv2[3], da2[3];
+                path_dv(m, dm, e, w, dt, ha, s1, q, dv1);
+                path_dL(m, v1, al, dm, dv1, q[2], da1);
+                for (int i = 0; i < 3; ++i) dmt[i] = dm[i] + da1[i];
+                path_dv(mt, dmt, e, w, dt, ha, s2, q, dv2);
+                path_dL(mt, v2, al, dmt, dv2, q[2], da2);
+                for (int i = 0; i < 3; ++i) dm[i] = dm[i] + 0.5 * (da1[i] + da2[i]);
+            }
+            for (int i = 0; i < 3; ++i) m[i] = m[i] + 0.5 * (a1[i] + a2[i]);
+        }
+        for (size_t d = 0; d < n_dirs; ++d) dmz_final_sum[d] += tm[3 * d + 2];
+    }
+    free(tm);
+}

@@ -180,6 +180,57 @@ int orc_full_dynamics(
  * integration steps the loop takes (k[S-1] + 1). */
 int64_t orc_sample_schedule(double dt, double end_time, size_t max_samples, int64_t *k);
 
+
+/* ---------------- pathwise sensitivities ----------------
+ * The differentiable Heun path of independent particles (one particle per
+ * member, k_red = v_red = 1).  Step n of particle p:
+ *   v1 = dt h(m, s_{n+1}) + sig w_n,     m~ = m + L(m, v1)
+ *   v2 = dt h(m~, s_{n+2}) + sig w_n,    m' = m + (L(m, v1) + L(m~, v2)) / 2
+ *   h(m, s) = (m.e) e + happ s z,  L(m, v) = -m x v + alpha (v |m|^2 - m (m.v))
+ *   s_k = sin(2 pi frac(k fdt)),  fdt = f * dt in SI (field cycles per step)
+ *   w_n = normals 3n, 3n+1, 3n+2 of the Philox stream (seed_p, particle 0, tag).
+ * This is integrator::heun (lib/integrators.cpp:372-405) called at
+ * t = (n+1) dt (lib/simulation.cpp:355, 372-375) with the drift and
+ * diffusion of lib/llg.cpp:14-29, 92-106 (a dt + B w sqrt(dt) = L(m, h dt +
+ * sr sqrt(dt) w)) and the fields of lib/field.cpp:37-40, 116-132; the phase
+ * of the sine is taken in SI cycles so that it does not move with the
+ * parameters.  The objective reads the ensemble sums of m_z at the steps
+ * sample_steps[s] (state after that many steps) and after n_steps. */
+typedef struct {
+    size_t n_particles;
+    const double *axis;          /* [P][3] easy axes */
+    const double *m0;            /* [P][3] initial state */
+    const double *dt;            /* [P] reduced step */
+    const double *sig;           /* [P] sr sqrt(dt) */
+    const double *alpha;         /* [P] damping */
+    const double *happ;          /* [P] reduced field amplitude */
+    double fdt;                  /* field cycles per step */
+    const uint64_t *seed;        /* [P] */
+    uint32_t tag;
+    int64_t n_steps;
+    const int64_t *sample_steps; /* [S], ascending, each in [0, n_steps] */
+    size_t n_samples;
+} orc_path_problem;
+
+/* sin(2 pi frac(k fdt)) */
+double orc_path_phase(double fdt, int64_t k);
+
+/* One step of the path above (m updated in place). */
+void orc_path_step(double m[3], const double e[3], const double w[3], double dt, double sig,
+                   double alpha, double happ, double s1, double s2);
+
+/* Forward run: mz_sum[s] = sum_p m_z,p(sample_steps[s]), *mz_final_sum =
+ * sum_p m_z,p(n_steps); final_state [P][3] may be NULL. */
+void orc_path_run(const orc_path_problem *pr, double *mz_sum, double *mz_final_sum,
+                  double *final_state);
+
+/* Forward-mode tangent of the run along n_dirs directions: dq[(p*n_dirs +
+ * d)*4 + j] is the derivative of particle p's (dt, sig, alpha, happ)[j]
+ * along direction d.  Writes dmz_sum[s*n_dirs + d] = d/dtheta_d of
+ * mz_sum[s] and dmz_final_sum[d]. */
+void orc_path_tangent(const orc_path_problem *pr, size_t n_dirs, const double *dq,
+                      double *dmz_sum, double *dmz_final_sum);
+
 #ifdef __cplusplus
 }
 #endif
