@@ -15,6 +15,11 @@ Other workloads (--workload):
   implicit  the 65,536-particle AC-field ensemble with the implicit midpoint
             scheme at dt = 5e-13 s (FP64, per-particle Newton); the run ends
             after the timed blocks so the Newton counts can be reported.
+  replicas  one 512-particle cluster x 4,096 thermal replicas, dipolar field
+            as an MFMA product (BF16 three-term split).
+  adjoint   16,384 AC-field particles, forward pass with checkpoints and the
+            reverse (adjoint) pass of the Heun step; gradient of a weighted
+            sum of <M_z> samples with respect to (K, alpha, H, median r).
 
 One bench "step" is one step block: BLOCK time steps of every particle, one
 kernel launch.  value = particle-steps / wall time of the timed blocks.  The
@@ -49,7 +54,7 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (published)
 # device copy; see profiles/heun_indep_fp32_65536_ac_field.md.
 TRAFFIC_BYTES_DEFAULT_LAUNCH = 13.30e6
 DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256),
-            "replicas": (4096, 64)}
+            "replicas": (4096, 64), "adjoint": (16384, 1 << 20)}
 
 
 def ac_field_inputs(n, seed=2024):
@@ -242,6 +247,145 @@ def sar_run(n, precision, save, stop=0, checkpoint="", resume=""):
                       "nonfinite": int(np.sum(res["status"] != 0))}), flush=True)
 
 
+# Arithmetic of the sensitivity kernels per particle-step, counted from
+# adjoint_kernels.hpp (add/sub/mul = 1, the FP64 gradient sums included):
+# one forward Heun step of the fused form is 118 FLOP; the reverse pass
+# recomputes the step (118) and runs its adjoint (325).
+FLOP_ADJ_FORWARD = 118
+FLOP_ADJ_REVERSE = 118 + 325
+
+
+def adjoint_bench(args):
+    """Gradient throughput of the differentiation workload: 16,384 particles
+    of the AC-field distribution, theta = (K, alpha, H, median r) at
+    (23 kJ/m^3, 0.1, 20 mT, 10 nm), checkpoints every 1,000 steps.  A bench
+    step is one block of time steps; the run is warmup + steps blocks long,
+    its forward pass is timed block by block and its reverse pass likewise
+    (the reverse's first `steps` blocks, which mirror the timed forward
+    blocks).  value = particle-steps / (forward + reverse wall time) of the
+    timed blocks."""
+    import magpy_amd
+    core = magpy_amd.core
+    P = args.particles or DEFAULTS["adjoint"][0]
+    B = args.block or DEFAULTS["adjoint"][1]
+    inp = ac_field_inputs(P)
+    shape = inp["radius"] / 10e-9
+    ax = inp["anisotropy_axis"]
+    theta = np.array([23e3, 0.1, 0.02 / MU0, 10e-9])
+    n_blocks = args.warmup + args.steps
+    N = B * n_blocks
+    ks = np.arange(0, N + 1, max(B // 8, 1), dtype=np.int64)
+    print("device:", core.device_info(0), flush=True)
+    os.environ["MAGPY_AMD_ADJOINT_STEPS_PER_LAUNCH"] = str(B)
+    run = core.Adjoint(shape, ax, ax, inp["seeds"], 400e3, 300.0, 3e5, 1e-14, N, ks, 1000, "fp32")
+    run.begin(theta)
+    t_fwd = 0.0
+    for b in range(n_blocks):
+        t0 = time.perf_counter()
+        run.forward_advance(B)
+        if b >= args.warmup:
+            t_fwd += time.perf_counter() - t0
+    ens, fin, status = run.forward_result()
+    w = np.linspace(-1.0, 1.0, len(ks))
+    run.reverse_begin(w, 1.0)
+    t_rev = 0.0
+    left = N
+    rev_blocks = 0
+    while left > 0:
+        t0 = time.perf_counter()
+        left = run.reverse_advance(B)
+        rev_blocks += 1
+        if rev_blocks <= args.steps:
+            t_rev += time.perf_counter() - t0
+    grad = run.reverse_result()
+    info = run.info()
+    ps = float(P) * B * args.steps
+    value = ps / (t_fwd + t_rev)
+    rev_rate = ps / t_rev
+    achieved = FLOP_ADJ_REVERSE * rev_rate / 1e12
+    line = {
+        "metric": METRIC, "value": value, "unit": "gradient particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": (t_fwd + t_rev) / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "differentiation: 16,384 AC-field particles (20 mT, 300 kHz, "
+                               "dt=1e-14 s), pathwise Heun adjoint, FP32 state, checkpoints "
+                               "every 1,000 steps; step = %d time steps forward + reverse" % B,
+                   "particles": P, "time_steps_per_step": B, "run_length_steps": N,
+                   "checkpoint_bytes": info["checkpoint_bytes"],
+                   "scratch_bytes": info["scratch_bytes"], "forward_status": int(status),
+                   "gradient_finite": bool(np.all(np.isfinite(grad)))},
+        "forward_particle_steps_per_s": ps / t_fwd, "reverse_particle_steps_per_s": rev_rate,
+        "roofline": {"bound": "valu", "kernel": "path_reverse_kernel<float>",
+                     "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
+                     "flop_per_particle_step": FLOP_ADJ_REVERSE,
+                     "forward_achieved": FLOP_ADJ_FORWARD * ps / t_fwd / 1e12,
+                     "device_ms_forward_all_blocks": info["forward_ms"],
+                     "device_ms_reverse_all_blocks": info["reverse_ms"]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = adjoint_cpu_baseline(inp, theta)
+        line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+def adjoint_cpu_baseline(inp, theta, seconds=12.0):
+    """The oracle's FP64 forward-mode tangent (4 directions) on the host
+    cores: gradient particle-steps/s on a bounded sample of the workload."""
+    from oracle import oracle as O
+    import magpy_amd
+    O.lib()
+    n_threads = min(os.cpu_count() or 1, 16)
+    steps, per = 20000, 4
+    shape = inp["radius"] / 10e-9
+    q, dq = magpy_amd.core.adjoint_reduced(theta, shape, 400e3, 300.0, 1e-14)
+    dirs = np.ascontiguousarray(np.transpose(dq, (0, 2, 1)))
+    P = len(shape)
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0}
+    t_stop = [0.0]
+
+    def chunk(i):
+        b = (i * per) % (P - per + 1)
+        sl = slice(b, b + per)
+        ax = inp["anisotropy_axis"][sl]
+        pr = O.path_problem(ax, ax, q[sl, 0], q[sl, 1], q[sl, 2], q[sl, 3], 3e5 * 1e-14,
+                            inp["seeds"][sl], steps, [0, steps])
+        O.path_tangent(pr, dirs[sl])
+        return per * steps
+
+    def worker():
+        while True:
+            with lock:
+                if time.perf_counter() > t_stop[0]:
+                    return
+                i = state["next"]
+                state["next"] += 1
+            ps = chunk(i)
+            with lock:
+                state["done"] += ps
+
+    def run(nt, secs):
+        state["next"], state["done"] = 0, 0
+        t0 = time.perf_counter()
+        t_stop[0] = t0 + secs
+        th = [threading.Thread(target=worker) for _ in range(nt)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return state["done"] / (time.perf_counter() - t0), state["next"]
+
+    one, m1 = run(1, seconds / 4)
+    allc, ma = run(n_threads, seconds)
+    return {"value": allc, "unit": "gradient particle-steps/s", "cores": n_threads,
+            "kind": "port", "value_1core": one,
+            "sample": "%d chunks of %d particles x %d steps of the FP64 oracle's forward-mode "
+                      "tangent (4 directions: dq/d(K, alpha, H, r)) on %d threads; 1-core rate "
+                      "from %d chunks" % (ma, per, steps, n_threads, m1)}
+
+
 def copy_calibration(nbytes):
     """One device-to-device hipMemcpy of nbytes (a blit kernel on ROCm), the
     known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters."""
@@ -293,6 +437,9 @@ def main():
               "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "adjoint":
+        adjoint_bench(args)
+        return
     import magpy_amd
     core = magpy_amd.core
     n = args.particles or DEFAULTS[args.workload][0]

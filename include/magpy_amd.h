@@ -259,6 +259,13 @@ int mp_adjoint_info(mp_adjoint *a, double *forward_ms, double *reverse_ms, int64
  * theta and their derivatives dq[p][j][i] = dq_j / dtheta_i (host only) */
 int mp_adjoint_reduced(const mp_adjoint_spec *spec, const double theta[4], double *q, double *dq);
 
+/* simulation::save_results (lib/simulation.cpp:38-63): writes prefix.mx,
+ * prefix.my, prefix.mz, prefix.field and prefix.time, each the raw
+ * native-endian FP64 array of n_samples values (io::write_array,
+ * include/io.hpp:13-26), for one particle's results. */
+int mp_save_results(const char *prefix, const double *mx, const double *my, const double *mz,
+                    const double *field, const double *time, size_t n_samples);
+
 /* get_KB / get_mu0 / get_gamma (magpy/core.pyx:29-34) */
 void mp_constants(double *kb, double *mu0, double *gamma);
 

@@ -111,6 +111,9 @@ cdef extern from "magpy_amd.h" nogil:
                         int64_t *checkpoint_bytes, int64_t *scratch_bytes)
     int mp_adjoint_reduced(const mp_adjoint_spec *spec, const double *theta, double *q,
                            double *dq)
+    int mp_save_results(const char *prefix, const double *mx, const double *my,
+                        const double *mz, const double *field, const double *time,
+                        size_t n_samples)
     void mp_constants(double *kb, double *mu0, double *gamma)
     const char *mp_last_error()
     const char *mp_version()
@@ -683,3 +686,18 @@ cdef class Adjoint:
         _check(mp_adjoint_info(self.a, &fms, &rms, &ckb, &scb))
         return {'forward_ms': fms, 'reverse_ms': rms, 'checkpoint_bytes': ckb,
                 'scratch_bytes': scb}
+
+
+def save_results(str prefix, mx, my, mz, field, time):
+    """simulation::save_results (lib/simulation.cpp:38-63): raw FP64 files
+    prefix.mx/.my/.mz/.field/.time of one particle's samples."""
+    arrs = [_c(a, np.float64).reshape(-1) for a in (mx, my, mz, field, time)]
+    n = arrs[0].shape[0]
+    if any(a.shape[0] != n for a in arrs):
+        raise ValueError("all five arrays need the same length")
+    if n == 0:
+        arrs = [np.zeros(1) for _ in arrs]
+    cdef const double[::1] a0 = arrs[0], a1 = arrs[1], a2 = arrs[2], a3 = arrs[3], a4 = arrs[4]
+    b = prefix.encode()
+    cdef const char *pb = b
+    _check(mp_save_results(pb, &a0[0], &a1[0], &a2[0], &a3[0], &a4[0], n))

@@ -1426,6 +1426,24 @@ void mp_constants(double *kb, double *mu0, double *gamma)
     if (gamma) *gamma = kGYROMAG;
 }
 
+int mp_save_results(const char *prefix, const double *mx, const double *my, const double *mz,
+                    const double *field, const double *time, size_t n_samples)
+{
+    return guarded([&] {
+        require(prefix && mx && my && mz && field && time, "prefix and all five arrays are required");
+        const char *ext[5] = {".mx", ".my", ".mz", ".field", ".time"};
+        const double *arr[5] = {mx, my, mz, field, time};
+        for (int i = 0; i < 5; ++i) {
+            const std::string name = std::string(prefix) + ext[i];
+            FILE *f = std::fopen(name.c_str(), "wb");
+            if (!f) throw simulation::Error(MP_ERR_ARG, "cannot open " + name + " for writing");
+            const size_t w = n_samples ? std::fwrite(arr[i], sizeof(double), n_samples, f) : 0;
+            const bool ok = std::fclose(f) == 0 && w == n_samples;
+            if (!ok) throw simulation::Error(MP_ERR_ARG, "failed to write " + name);
+        }
+    });
+}
+
 const char *mp_last_error(void) { return last_error().c_str(); }
 
 const char *mp_version(void) { return "magpy_amd 0.1 (abi 1, gfx950)"; }

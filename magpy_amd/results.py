@@ -68,6 +68,20 @@ class Results:
         }
 
 
+def save_results(prefix, results, particle=0):
+    """Write one particle of a Results as simulation::save_results does
+    (lib/simulation.cpp:38-63): raw FP64 prefix.mx, .my, .mz, .field, .time."""
+    core.save_results(prefix, results.x[particle], results.y[particle], results.z[particle],
+                      results.field, results.time)
+
+
+def load_results(prefix):
+    """Read the five files of save_results back into a one-particle Results."""
+    arr = {ext: np.fromfile(prefix + "." + ext, dtype=np.float64)
+           for ext in ("mx", "my", "mz", "field", "time")}
+    return Results(arr["time"], arr["field"], {0: arr["mx"]}, {0: arr["my"]}, {0: arr["mz"]}, 1)
+
+
 class _BatchResults:
     """Lazy sequence of per-member Results over one batched run."""
 

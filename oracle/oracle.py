@@ -155,6 +155,8 @@ def ref():
         R.ref_field_multi_add_dipolar.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp, _dp,
                                                   _dp, C.c_size_t]
         R.ref_distances.argtypes = [_dp, C.c_size_t, _dp, _dp]
+        R.ref_write_array.restype = C.c_int
+        R.ref_write_array.argtypes = [C.c_char_p, _dp, C.c_size_t]
         _ref = R
     return _ref
 

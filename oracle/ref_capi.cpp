@@ -2,13 +2,18 @@
 //
 // TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
 // the reference's lib/llg.cpp, lib/field.cpp, lib/rng.cpp, lib/distances.cpp
-// and lib/trap.cpp, read where they lie, into oracle/_ref/libmagpy_ref.so.
+// and lib/trap.cpp, read where they lie, into oracle/_ref/libmagpy_ref.so
+// (include/io.hpp's write_array is header-only and comes along).
 // The tests use it to check the oracle's restatement of those functions.
 #include "llg.hpp"
 #include "field.hpp"
 #include "rng.hpp"
 #include "distances.hpp"
 #include "trap.hpp"
+// io.hpp uses std::runtime_error and FILE without including their headers
+#include <cstdio>
+#include <stdexcept>
+#include "io.hpp"
 #include <array>
 #include <vector>
 
@@ -105,6 +110,12 @@ void ref_distances(const double *pts, size_t n, double *units, double *mags)
             mags[i * n + j] = m[i][j];
             for (size_t k = 0; k < 3; ++k) units[(i * n + j) * 3 + k] = u[i][j][k];
         }
+}
+
+// io::write_array (include/io.hpp:13-26), the writer of simulation::save_results
+int ref_write_array(const char *fname, const double *arr, size_t len)
+{
+    return io::write_array(std::string(fname), arr, len);
 }
 
 }  // extern "C"
