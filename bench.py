@@ -197,7 +197,7 @@ def cpu_baseline(name, inp, seconds=12.0):
                                                                      n_threads, m1)}
 
 
-def sar_run(n, precision, save, stop=0, checkpoint="", resume=""):
+def sar_run(n, precision, save, stop=0, checkpoint="", resume="", noise_tag=0):
     """The full AC-field workload (5 cycles) in one precision; prints progress
     and the final-cycle hysteresis energy through EnsembleResults.
 
@@ -218,7 +218,8 @@ def sar_run(n, precision, save, stop=0, checkpoint="", resume=""):
         print(json.dumps({"resume_from_step": start["start_step"]}), flush=True)
     ses = core.Session(precision=precision, trajectories=False, ensemble_sum=True,
                        time_step=1e-14, end_time=cycles / f, max_samples=2000 * cycles + 1,
-                       interactions=False, use_implicit=False, **start, **inp)
+                       interactions=False, use_implicit=False, noise_tag=noise_tag, **start,
+                       **inp)
     total = ses.info()["n_steps"]
     upto = min(stop, total) if stop else total
     t0 = time.perf_counter()
@@ -241,7 +242,8 @@ def sar_run(n, precision, save, stop=0, checkpoint="", resume=""):
     if save:
         np.savez(save, time=res["time"], field=res["field"], ensemble_m=res["ensemble_m"],
                  n_members=n)
-    print(json.dumps({"sar_run": precision, "particles": n, "steps": int(res["n_steps"]),
+    print(json.dumps({"sar_run": precision, "noise_tag": noise_tag, "particles": n,
+                      "steps": int(res["n_steps"]),
                       "final_cycle_energy_J_per_m3": energy, "sar_W_per_m3": energy * f,
                       "wall_s": el, "kernel_ms": res["kernel_ms"],
                       "nonfinite": int(np.sum(res["status"] != 0))}), flush=True)
@@ -424,13 +426,15 @@ def main():
                     help="end the --sar-run at this step and write --sar-checkpoint")
     ap.add_argument("--sar-checkpoint", default="", help="npz written at --sar-stop")
     ap.add_argument("--sar-resume", default="", help="npz checkpoint to resume --sar-run from")
+    ap.add_argument("--noise-tag", type=int, default=0,
+                    help="Philox stream tag of the --sar-run (another thermal history)")
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
         return
     if args.sar_run:
         sar_run(args.particles or 65536, args.sar_run, args.save, args.sar_stop,
-                args.sar_checkpoint, args.sar_resume)
+                args.sar_checkpoint, args.sar_resume, args.noise_tag)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build's path runs on a single MI355X "

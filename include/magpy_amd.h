@@ -259,6 +259,43 @@ int mp_adjoint_info(mp_adjoint *a, double *forward_ms, double *reverse_ms, int64
  * theta and their derivatives dq[p][j][i] = dq_j / dtheta_i (host only) */
 int mp_adjoint_reduced(const mp_adjoint_spec *spec, const double theta[4], double *q, double *dq);
 
+/* ------------------------------------------------ discrete orientation model
+ * simulation::dom_ensemble_dynamics (lib/simulation.cpp:659-767) as
+ * core.simulate_dom calls it (magpy/core.pyx:205-262), for every particle of
+ * an ensemble in one device launch: a two-state master equation with
+ * Neel-Brown rates (lib/dom.cpp:30-93) integrated by adaptive Cash-Karp
+ * RK45 (lib/integrators.cpp:152-251, tolerance = time_step), sampled at
+ * max_samples regular times by first-order hold.  The field is applied along
+ * each particle's axis; its amplitude (A/m) is reduced by each particle's
+ * H_k = 2K / (mu0 Ms).  MP_FIELD_SQUARE_FOURIER is core.simulate_dom's
+ * 'square_f' (field::square_fourier, lib/field.cpp:67-78). */
+#define MP_FIELD_SQUARE_FOURIER 3
+typedef struct mp_dom_spec {
+    int64_t n_particles;
+    const double *volume;                 /* [P] m^3 */
+    const double *anisotropy;             /* [P] J/m^3 */
+    const double *temperature;            /* [P] K */
+    const double *magnetisation;          /* [P] A/m */
+    const double *damping;                /* [P] */
+    const double *initial_probabilities;  /* [P][2]: (up, down) */
+    int32_t field_shape;                  /* MP_FIELD_* or MP_FIELD_SQUARE_FOURIER */
+    double field_amplitude;               /* A/m */
+    double field_frequency;               /* Hz */
+    int64_t field_n_components;           /* square Fourier terms */
+    double time_step;                     /* s: initial step / 100 and RK45 tolerance */
+    double end_time;                      /* s */
+    int64_t max_samples;                  /* >= 2 */
+    int64_t max_steps;                    /* per particle; <= 0: 2^32.  A particle that
+                                             reaches it (or a step that is never
+                                             accepted) gets status 1 and NaN samples */
+    int32_t device;
+} mp_dom_spec;
+/* time [S] (s), field [S] (A/m, particle 0's reduced field times its H_k, as
+ * the reference returns it), mz [P][S] = p_up - p_down, status [P],
+ * steps [P] accepted RK45 steps; any output may be NULL */
+int mp_dom_run(const mp_dom_spec *spec, double *time, double *field, double *mz,
+               uint32_t *status, int64_t *steps);
+
 /* simulation::save_results (lib/simulation.cpp:38-63): writes prefix.mx,
  * prefix.my, prefix.mz, prefix.field and prefix.time, each the raw
  * native-endian FP64 array of n_samples values (io::write_array,

@@ -5,7 +5,7 @@ The compute path is libmagpy_amd.so (HIP, gfx950) reached through the Cython
 binding magpy_amd.core; importing this package fails if either is missing.
 """
 from . import core
-from .model import EnsembleModel, Model
+from .model import DOModel, EnsembleModel, Model
 from .results import EnsembleResults, Results
 
-__all__ = ["core", "Model", "EnsembleModel", "Results", "EnsembleResults"]
+__all__ = ["core", "Model", "EnsembleModel", "DOModel", "Results", "EnsembleResults"]

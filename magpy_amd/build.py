@@ -3,7 +3,8 @@
   python -m magpy_amd.build        # or magpy_amd.build.build()
 
 1. libmagpy_amd.so   hipcc, gfx950, from csrc/ensemble_run.hip and
-                     csrc/adjoint.hip (kernels in csrc/*_kernels.hpp);
+                     csrc/adjoint.hip and csrc/dom.hip (kernels in
+                     csrc/*_kernels.hpp);
                      exports the C ABI of include/magpy_amd.h.
 2. core.*.so         Cython binding core.pyx, linked against libmagpy_amd.so
                      with an $ORIGIN rpath.
@@ -39,7 +40,7 @@ def core_ext_path():
     return os.path.join(HERE, "core" + suffix)
 
 
-SOURCES = ["ensemble_run.hip", "adjoint.hip"]
+SOURCES = ["ensemble_run.hip", "adjoint.hip", "dom.hip"]
 
 
 def build_lib():

@@ -111,6 +111,25 @@ cdef extern from "magpy_amd.h" nogil:
                         int64_t *checkpoint_bytes, int64_t *scratch_bytes)
     int mp_adjoint_reduced(const mp_adjoint_spec *spec, const double *theta, double *q,
                            double *dq)
+    ctypedef struct mp_dom_spec:
+        int64_t n_particles
+        const double *volume
+        const double *anisotropy
+        const double *temperature
+        const double *magnetisation
+        const double *damping
+        const double *initial_probabilities
+        int32_t field_shape
+        double field_amplitude
+        double field_frequency
+        int64_t field_n_components
+        double time_step
+        double end_time
+        int64_t max_samples
+        int64_t max_steps
+        int32_t device
+    int mp_dom_run(const mp_dom_spec *spec, double *time, double *field, double *mz,
+                   uint32_t *status, int64_t *steps)
     int mp_save_results(const char *prefix, const double *mx, const double *my,
                         const double *mz, const double *field, const double *time,
                         size_t n_samples)
@@ -701,3 +720,80 @@ def save_results(str prefix, mx, my, mz, field, time):
     b = prefix.encode()
     cdef const char *pb = b
     _check(mp_save_results(pb, &a0[0], &a1[0], &a2[0], &a3[0], &a4[0], n))
+
+
+DOM_FIELD_SHAPES = {'sine': 0, 'square': 1, 'constant': 2, 'square_f': 3}
+
+
+def simulate_dom_ensemble(volume, anisotropy, temperature, magnetisation, damping,
+                          initial_probabilities, double time_step, double end_time,
+                          int64_t max_samples, str field_shape='constant',
+                          double field_amplitude=0.0, double field_frequency=0.0,
+                          int64_t field_n_components=1, int64_t max_steps=0, int device=0):
+    """Discrete orientation model of every particle of an ensemble in one
+    device launch (mp_dom_run).  Per-particle arrays (length P); returns
+    dict(time [S], field [S] (A/m), mz [P, S], status [P], steps [P])."""
+    v = _c(volume, np.float64).reshape(-1)
+    P = v.shape[0]
+    if P == 0:
+        raise ValueError("no particles")
+    per = [_c(np.broadcast_to(np.asarray(x, dtype=np.float64), (P,)), np.float64)
+           for x in (anisotropy, temperature, magnetisation, damping)]
+    ip = _c(np.broadcast_to(np.asarray(initial_probabilities, dtype=np.float64), (P, 2)),
+            np.float64)
+    if field_shape not in DOM_FIELD_SHAPES:
+        raise ValueError("field_shape must be one of %s" % sorted(DOM_FIELD_SHAPES))
+    cdef const double[::1] v_v = v, k_v = per[0], t_v = per[1], m_v = per[2], a_v = per[3]
+    cdef const double[:, ::1] ip_v = ip
+    cdef mp_dom_spec sp
+    sp.n_particles = P
+    sp.volume = &v_v[0]
+    sp.anisotropy = &k_v[0]
+    sp.temperature = &t_v[0]
+    sp.magnetisation = &m_v[0]
+    sp.damping = &a_v[0]
+    sp.initial_probabilities = &ip_v[0, 0]
+    sp.field_shape = DOM_FIELD_SHAPES[field_shape]
+    sp.field_amplitude = field_amplitude
+    sp.field_frequency = field_frequency
+    sp.field_n_components = field_n_components
+    sp.time_step = time_step
+    sp.end_time = end_time
+    sp.max_samples = max_samples
+    sp.max_steps = max_steps
+    sp.device = device
+    S = max(max_samples, 1)
+    time = np.zeros(S)
+    field = np.zeros(S)
+    mz = np.zeros((P, S))
+    status = np.zeros(P, dtype=np.uint32)
+    steps = np.zeros(P, dtype=np.int64)
+    cdef double[::1] tv = time, fv = field
+    cdef double[:, ::1] mzv = mz
+    cdef uint32_t[::1] sv = status
+    cdef int64_t[::1] stv = steps
+    cdef int rc
+    with nogil:
+        rc = mp_dom_run(&sp, &tv[0], &fv[0], &mzv[0, 0], &sv[0], &stv[0])
+    _check(rc)
+    return {'time': time, 'field': field, 'mz': mz, 'status': status, 'steps': steps}
+
+
+cpdef simulate_dom(initial_probabilities, double volume, double anisotropy, double temperature,
+                   double magnetisation, double alpha, double time_step, double end_time,
+                   size_t max_samples, str field_shape, double field_amplitude,
+                   double field_frequency, size_t field_n_components):
+    """core.simulate_dom (magpy/core.pyx:205-262): one particle's state
+    probabilities; the result dictionary of the reference (mz in 'z', x and y
+    zero)."""
+    r = simulate_dom_ensemble([volume], [anisotropy], [temperature], [magnetisation], [alpha],
+                              np.reshape(np.asarray(initial_probabilities, dtype=np.float64),
+                                         (1, 2)),
+                              time_step, end_time, max_samples, field_shape, field_amplitude,
+                              field_frequency, field_n_components)
+    if r['status'][0]:
+        raise FloatingPointError("the master equation did not integrate (non-finite rates or "
+                                 "no accepted step)")
+    S = max_samples
+    return {'N': 1, 'time': r['time'], 'field': r['field'], 'x': {0: np.zeros(S)},
+            'y': {0: np.zeros(S)}, 'z': {0: r['mz'][0]}}

@@ -142,3 +142,36 @@ class EnsembleModel:
                 if not np.array_equal(np.asarray(p[k]), np.asarray(first[k])):
                     return True
         return False
+
+
+class DOModel:
+    """Two-state (up/down) probabilistic model of one particle with uniaxial
+    anisotropy (magpy/model.py:211-300): the state probabilities obey a
+    master equation with Neel-Brown rates in a field applied along the
+    anisotropy axis ('constant', 'sine', 'square' or 'square_f' with
+    field_n_components Fourier terms)."""
+
+    def __init__(self, radius, anisotropy, initial_probabilities, magnetisation, damping,
+                 temperature, field_shape='constant', field_frequency=0.0, field_amplitude=0.0,
+                 field_n_components=1):
+        self.radius = radius
+        self.volume = 4. / 3 * np.pi * self.radius ** 3
+        self.anisotropy = anisotropy
+        self.initial_probabilities = np.array(initial_probabilities)
+        self.magnetisation = magnetisation
+        self.damping = damping
+        self.temperature = temperature
+        self.field_shape = field_shape
+        self.field_frequency = field_frequency
+        self.field_amplitude = field_amplitude
+        self.field_n_components = field_n_components
+
+    def simulate(self, end_time, time_step, max_samples):
+        """Probabilities sampled at max_samples regular times (first-order
+        hold) as a Results whose z holds p_up - p_down (magpy/model.py:262-300)."""
+        results = core.simulate_dom(
+            self.initial_probabilities, self.volume, self.anisotropy, self.temperature,
+            self.magnetisation, self.damping, time_step, end_time, max_samples,
+            self.field_shape, self.field_amplitude, self.field_frequency,
+            self.field_n_components)
+        return Results(**results)

@@ -31,6 +31,8 @@ RNG_MT, RNG_ARRAY, RNG_PHILOX = 0, 1, 2
 SDE_FN = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp, _dp, C.c_double)
 SDEJAC_FN = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double)
 FJ_FN = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp, _dp)
+ODE_FN = C.CFUNCTYPE(None, C.c_void_p, _dp, _dp, C.c_double)
+FIELD_SQUARE_F = 3
 
 
 class PathProblem(C.Structure):
@@ -117,6 +119,19 @@ def lib():
         L.orc_path_run.argtypes = [C.POINTER(PathProblem), _dp, _dp, _dp]
         L.orc_path_tangent.restype = None
         L.orc_path_tangent.argtypes = [C.POINTER(PathProblem), C.c_size_t, _dp, _dp, _dp]
+        L.orc_field_square_fourier.restype = C.c_double
+        L.orc_field_square_fourier.argtypes = [C.c_double, C.c_double, C.c_double, C.c_size_t]
+        L.orc_dom_transition_matrix.restype = None
+        L.orc_dom_transition_matrix.argtypes = [_dp] + [C.c_double] * 6
+        L.orc_master_equation.restype = None
+        L.orc_master_equation.argtypes = [_dp, _dp, _dp, C.c_size_t]
+        L.orc_rk45.restype = None
+        L.orc_rk45.argtypes = [_dp] * 10 + [_dp, ODE_FN, C.c_void_p, C.c_size_t, C.c_double]
+        L.orc_dom_ensemble_dynamics.restype = None
+        L.orc_dom_ensemble_dynamics.argtypes = [C.c_double] * 5 + [C.c_int, C.c_double,
+                                                                  C.c_double, C.c_size_t, _dp,
+                                                                  C.c_double, C.c_double,
+                                                                  C.c_size_t, _dp, _dp, _dp]
         L.orc_sample_schedule.restype = C.c_int64
         L.orc_sample_schedule.argtypes = [C.c_double, C.c_double, C.c_size_t, _i64p]
         L.orc_full_dynamics.restype = C.c_int
@@ -155,6 +170,8 @@ def ref():
         R.ref_field_multi_add_dipolar.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp, _dp,
                                                   _dp, C.c_size_t]
         R.ref_distances.argtypes = [_dp, C.c_size_t, _dp, _dp]
+        R.ref_field_square_fourier.restype = C.c_double
+        R.ref_field_square_fourier.argtypes = [C.c_double, C.c_double, C.c_double, C.c_size_t]
         R.ref_write_array.restype = C.c_int
         R.ref_write_array.argtypes = [C.c_char_p, _dp, C.c_size_t]
         _ref = R
@@ -299,3 +316,37 @@ def path_tangent(pr, dq):
     fin = np.zeros(D)
     lib().orc_path_tangent(C.byref(pr[0]), D, ptr(dq), ptr(out) if S else None, ptr(fin))
     return out, fin
+
+
+# ---------------------------------------------------------------- discrete orientation model
+def rk45(ode, x, t, h, tol):
+    """One adaptive step of orc_rk45 with a Python ODE f(x, t) -> dx.
+    Returns (next_state, k1..k6, t_new, h_new)."""
+    n = len(x)
+    x = f64(x)
+
+    def cb(_ctx, dx, xx, tt):
+        v = ode(np.ctypeslib.as_array(xx, shape=(n,)).copy(), tt)
+        for i in range(n):
+            dx[i] = v[i]
+
+    fn = ODE_FN(cb)
+    nxt, tmp = np.zeros(n), np.zeros(n)
+    ks = [np.zeros(n) for _ in range(6)]
+    hp, tp = C.c_double(h), C.c_double(t)
+    lib().orc_rk45(ptr(nxt), ptr(tmp), *[ptr(k) for k in ks], C.byref(hp), C.byref(tp),
+                   ptr(x), fn, None, n, tol)
+    return nxt, ks, tp.value, hp.value
+
+
+def dom_ensemble_dynamics(volume, anisotropy, temperature, magnetisation, alpha, field_shape,
+                          h_reduced, frequency, initial_probs, time_step, end_time, max_samples,
+                          n_components=1):
+    """orc_dom_ensemble_dynamics -> dict(time, field (reduced), mz)."""
+    S = int(max_samples)
+    time, field, mz = np.zeros(S), np.zeros(S), np.zeros(S)
+    shape = {"sine": 0, "square": 1, "constant": 2, "square_f": FIELD_SQUARE_F}[field_shape]
+    lib().orc_dom_ensemble_dynamics(volume, anisotropy, temperature, magnetisation, alpha, shape,
+                                    h_reduced, frequency, n_components, ptr(f64(initial_probs)),
+                                    time_step, end_time, S, ptr(time), ptr(field), ptr(mz))
+    return dict(time=time, field=field, mz=mz)

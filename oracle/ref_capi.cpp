@@ -118,4 +118,9 @@ int ref_write_array(const char *fname, const double *arr, size_t len)
     return io::write_array(std::string(fname), arr, len);
 }
 
+double ref_field_square_fourier(double t, double h, double f, size_t n)
+{
+    return field::square_fourier(t, h, f, n);
+}
+
 }  // extern "C"

@@ -1058,3 +1058,165 @@ void orc_path_tangent(const orc_path_problem *pr, size_t n_dirs, const double *d
     }
     free(tm);
 }
+
+/* ======================================================================
+ * Discrete orientation model
+ * ==================================================================== */
+double orc_field_square_fourier(double t, double h, double f, size_t n_components)
+{
+    double field = 0.0;
+    for (size_t k = 1; k < n_components + 1; ++k) {
+        double odd = (double)(2 * k - 1);
+        field += sin(2 * M_PI * odd * f * t) / odd;
+    }
+    field *= 4 / M_PI * h;
+    return field;
+}
+
+void orc_dom_transition_matrix(double *W, double k, double v, double T, double h, double ms,
+                               double alpha)
+{
+    double sigma = k * v / ORC_KB / T;
+    double taun = v * ms * (1 + alpha * alpha) / 2.0 / ORC_GYROMAG / alpha / ORC_KB / T;
+    double e1 = sigma * (1 - h) * (1 - h);
+    double e2 = sigma * (1 + h) * (1 + h);
+    double pre = taun * sqrt(M_PI) / pow(sigma, 1.5) / (1 - h * h);
+    double r1 = 1.0 / pre * (1 - h) * exp(-e1);
+    double r2 = 1.0 / pre * (1 + h) * exp(-e2);
+    W[0] = -r2;
+    W[1] = r1;
+    W[2] = r2;
+    W[3] = -r1;
+}
+
+void orc_master_equation(double *derivs, const double *W, const double *x, size_t dim)
+{
+    for (size_t i = 0; i < dim; ++i) {
+        double acc = 0.0;
+        for (size_t j = 0; j < dim; ++j) acc += W[i * dim + j] * x[j];
+        derivs[i] = acc;
+    }
+}
+
+/* Cash-Karp coefficients, include/integrators.hpp:128-160 */
+static const double CK_C11 = 0.2, CK_C21 = 3.0 / 40.0, CK_C22 = 9.0 / 40.0, CK_C31 = 3.0 / 10.0,
+                    CK_C32 = -9.0 / 10.0, CK_C33 = 6.0 / 5.0, CK_C41 = -11.0 / 54.0,
+                    CK_C42 = 2.5, CK_C43 = -70.0 / 27.0, CK_C44 = 35.0 / 27.0,
+                    CK_C51 = 1631.0 / 55296.0, CK_C52 = 175.0 / 512.0, CK_C53 = 575.0 / 13824.0,
+                    CK_C54 = 44275.0 / 110592.0, CK_C55 = 253.0 / 4096.0;
+static const double CK_H1 = 0.2, CK_H2 = 0.3, CK_H3 = 0.6, CK_H4 = 1.0, CK_H5 = 7.0 / 8.0;
+static const double CK_X11 = 37.0 / 378.0, CK_X13 = 250.0 / 621.0, CK_X14 = 125.0 / 594.0,
+                    CK_X16 = 512.0 / 1771.0, CK_X21 = 2825.0 / 27648.0,
+                    CK_X23 = 18575.0 / 48384.0, CK_X24 = 13525.0 / 55296.0,
+                    CK_X25 = 277.0 / 14336.0, CK_X26 = 0.25;
+
+void orc_rk45(double *next_state, double *temp_state, double *k1, double *k2, double *k3,
+              double *k4, double *k5, double *k6, double *h_ptr, double *t_ptr,
+              const double *x, orc_ode_fn derivs, void *ctx, size_t n, double tol)
+{
+    int ok = 0;
+    double err = 0.0, h = *h_ptr, t = *t_ptr;
+    while (!ok) {
+        derivs(ctx, k1, x, t);
+        for (size_t i = 0; i < n; ++i) next_state[i] = k1[i] * h * CK_C11 + x[i];
+        derivs(ctx, k2, next_state, t + h * CK_H1);
+        for (size_t i = 0; i < n; ++i) next_state[i] = x[i] + h * (CK_C21 * k1[i] + CK_C22 * k2[i]);
+        derivs(ctx, k3, next_state, t + h * CK_H2);
+        for (size_t i = 0; i < n; ++i)
+            next_state[i] = x[i] + h * (CK_C31 * k1[i] + CK_C32 * k2[i] + CK_C33 * k3[i]);
+        derivs(ctx, k4, next_state, t + h * CK_H3);
+        for (size_t i = 0; i < n; ++i)
+            next_state[i] = x[i] + h * (CK_C41 * k1[i] + CK_C42 * k2[i] + CK_C43 * k3[i] +
+                                        CK_C44 * k4[i]);
+        derivs(ctx, k5, next_state, t + h * CK_H4);
+        for (size_t i = 0; i < n; ++i)
+            next_state[i] = x[i] + h * (CK_C51 * k1[i] + CK_C52 * k2[i] + CK_C53 * k3[i] +
+                                        CK_C54 * k4[i] + CK_C55 * k5[i]);
+        derivs(ctx, k6, next_state, t + h * CK_H5);
+        for (size_t i = 0; i < n; ++i)
+            temp_state[i] = x[i] + h * (CK_X11 * k1[i] + CK_X13 * k3[i] + CK_X14 * k4[i] +
+                                        CK_X16 * k6[i]);
+        for (size_t i = 0; i < n; ++i)
+            next_state[i] = x[i] + h * (CK_X21 * k1[i] + CK_X23 * k3[i] + CK_X24 * k4[i] +
+                                        CK_X25 * k5[i] + CK_X26 * k6[i]);
+        err = 0.0;
+        double mag = 0.0;
+        for (size_t i = 0; i < n; ++i) mag += x[i] * x[i];
+        mag = pow(mag, 0.5);
+        for (size_t i = 0; i < n; ++i) err += pow(fabs(temp_state[i] - next_state[i]), 2);
+        err = pow(err, 0.5);
+        err /= ((double)n * tol * (1 + mag));
+        if (err < 1.0) {
+            ok = 1;
+        } else {
+            double f = 0.84 * pow(err, -0.2);
+            f = fabs(f) < 0.1 ? 0.1 : f;
+            h *= f;
+        }
+    }
+    *t_ptr = t + h;
+    double f = err == 0.0 ? 5.0 : 0.84 * pow(err, -0.2);
+    f = f > 5 ? 5.0 : f;
+    *h_ptr = f * h;
+}
+
+typedef struct {
+    double k, v, T, ms, alpha, h, f;
+    size_t ncomp;
+    int shape;
+} orc_dom_ctx;
+
+static double orc_dom_field(const orc_dom_ctx *c, double t)
+{
+    switch (c->shape) {
+    case ORC_FIELD_SINE: return orc_field_sinusoidal(t, c->h, c->f);
+    case ORC_FIELD_SQUARE: return orc_field_square(t, c->h, c->f);
+    case ORC_FIELD_SQUARE_F: return orc_field_square_fourier(t, c->h, c->f, c->ncomp);
+    default: return c->h;
+    }
+}
+
+/* dom::master_equation_with_update (lib/dom.cpp:80-93) */
+static void orc_dom_master(void *ctx, double *dx, const double *x, double t)
+{
+    const orc_dom_ctx *c = (const orc_dom_ctx *)ctx;
+    double W[4];
+    orc_dom_transition_matrix(W, c->k, c->v, c->T, orc_dom_field(c, t), c->ms, c->alpha);
+    orc_master_equation(dx, W, x, 2);
+}
+
+void orc_dom_ensemble_dynamics(double volume, double anisotropy, double temperature,
+                               double magnetisation, double alpha, int field_shape,
+                               double h_reduced, double frequency, size_t n_components,
+                               const double initial_probs[2], double time_step, double end_time,
+                               size_t max_samples, double *time, double *field, double *mz)
+{
+    orc_dom_ctx c = {anisotropy, volume, temperature, magnetisation, alpha, h_reduced, frequency,
+                     n_components, field_shape};
+    double last[2], next[2] = {initial_probs[0], initial_probs[1]};
+    double k1[2], k2[2], k3[2], k4[2], k5[2], k6[2], tmp[2];
+    const size_t S = max_samples;
+    const double ts = end_time / (double)(S - 1);
+    mz[0] = next[0] - next[1];
+    time[0] = 0;
+    field[0] = orc_dom_field(&c, 0);
+    double t = 0, t_last = 0;
+    const double max_dt = end_time / 1000.0;
+    double dt = 0.01 * time_step;
+    for (size_t s = 1; s < S; ++s) {
+        while (t <= (double)s * ts) {
+            last[0] = next[0];
+            last[1] = next[1];
+            t_last = t;
+            orc_rk45(next, tmp, k1, k2, k3, k4, k5, k6, &dt, &t, last, orc_dom_master, &c, 2,
+                     time_step);
+            dt = dt > max_dt ? max_dt : dt;
+        }
+        double mz_last = last[0] - last[1], mz_next = next[0] - next[1];
+        double ts_s = (double)s * ts;
+        double beta = (mz_next - mz_last) / (t - t_last);
+        mz[s] = mz_last + beta * (ts_s - t_last);
+        time[s] = ts_s;
+        field[s] = orc_dom_field(&c, ts_s);
+    }
+}

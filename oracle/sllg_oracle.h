@@ -231,6 +231,34 @@ void orc_path_run(const orc_path_problem *pr, double *mz_sum, double *mz_final_s
 void orc_path_tangent(const orc_path_problem *pr, size_t n_dirs, const double *dq,
                       double *dmz_sum, double *dmz_final_sum);
 
+
+/* ---------------- discrete orientation model (two-state master equation) ---------------- */
+/* field::square_fourier (lib/field.cpp:67-78) */
+double orc_field_square_fourier(double t, double h, double f, size_t n_components);
+/* dom::transition_matrix (lib/dom.cpp:30-56): Neel-Brown rates, W row-major 2x2 */
+void orc_dom_transition_matrix(double *W, double k, double v, double T, double h, double ms,
+                               double alpha);
+/* stochastic::master_equation (lib/stochastic_processes.cpp:18-25): derivs = W x */
+void orc_master_equation(double *derivs, const double *W, const double *x, size_t dim);
+/* integrator::rk45 (lib/integrators.cpp:152-251), Cash-Karp tableau
+ * (include/integrators.hpp:128-160), with the ODE as a callback */
+typedef void (*orc_ode_fn)(void *ctx, double *dx, const double *x, double t);
+void orc_rk45(double *next_state, double *temp_state, double *k1, double *k2, double *k3,
+              double *k4, double *k5, double *k6, double *h_ptr, double *t_ptr,
+              const double *current_state, orc_ode_fn derivs, void *ctx, size_t n_dims,
+              double tol);
+/* simulation::dom_ensemble_dynamics (lib/simulation.cpp:659-767) with the
+ * reduced field h(t) of core.simulate_dom (magpy/core.pyx:205-249):
+ * field_shape ORC_FIELD_* or ORC_FIELD_SQUARE_F (3), amplitude already
+ * reduced by H_k.  Writes time, reduced field and mz = p0 - p1 at the
+ * max_samples regular samples (first-order hold). */
+#define ORC_FIELD_SQUARE_F 3
+void orc_dom_ensemble_dynamics(double volume, double anisotropy, double temperature,
+                               double magnetisation, double alpha, int field_shape,
+                               double h_reduced, double frequency, size_t n_components,
+                               const double initial_probs[2], double time_step, double end_time,
+                               size_t max_samples, double *time, double *field, double *mz);
+
 #ifdef __cplusplus
 }
 #endif

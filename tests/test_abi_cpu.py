@@ -73,3 +73,18 @@ def test_errors_are_reported_not_raised_through_the_abi():
         magpy_amd.core.simulate_ensemble([0, 1], [1e-8], [2.3e4], [[0, 0, 1]], [[0, 0, 1]],
                                          [[0, 0, 0]], 4e5, 0.1, 300, 'constant', 0.0, 0.0, [1],
                                          1e-14, 1e-12, 4, start_step=-5)
+
+
+def test_dom_arguments_are_checked_before_the_device():
+    import magpy_amd
+    bad = [dict(volume=[-1e-24]), dict(max_samples=1), dict(field_shape="triangle"),
+           dict(field_shape="square_f", field_n_components=0), dict(time_step=0.0)]
+    base = dict(volume=[1e-24], anisotropy=23e3, temperature=300.0, magnetisation=4e5,
+                damping=0.1, initial_probabilities=[[1.0, 0.0]], time_step=1e-12,
+                end_time=1e-9, max_samples=5, field_shape="sine", field_amplitude=1e4,
+                field_frequency=3e5)
+    for b in bad:
+        kw = dict(base)
+        kw.update(b)
+        with pytest.raises(ValueError):
+            magpy_amd.core.simulate_dom_ensemble(**kw)
