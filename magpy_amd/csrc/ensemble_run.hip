@@ -505,9 +505,9 @@ struct GroupRun : GroupBase {
 int pad_to(int64_t x, int64_t q) { return (int)((x + q - 1) / q * q); }
 
 // Dipole tensor of one cluster in reduced units (lib/field.cpp:187-225,
-// lib/simulation.cpp:577-583): D[3i+a][3j+b] = w_j r^-3 (3 u_a u_b - d_ab),
-// w_j = mu0 Ms^2 / (8 pi K_avg) v_j / V_avg, row-major with leading dimension
-// ld, zero padded.
+// lib/simulation.cpp:577-583): D[(i,a)][(j,b)] = w_j r^-3 (3 u_a u_b - d_ab),
+// w_j = mu0 Ms^2 / (8 pi K_avg) v_j / V_avg, rows and columns in the
+// replica_col order, leading dimension ld, zero padded.
 void dipole_tensor(const simulation::ReducedMember &r, const double *loc, double ms, size_t n,
                    int ld, std::vector<double> &D)
 {
@@ -526,7 +526,7 @@ void dipole_tensor(const simulation::ReducedMember &r, const double *loc, double
             const double c = pre * r.v_red[j] / (rr * rr * rr);
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b)
-                    D[(3 * i + a) * (size_t)ld + 3 * j + b] =
+                    D[(size_t)mp::replica_col((int)i, a) * ld + mp::replica_col((int)j, b)] =
                         c * (3 * (d[a] / rr) * (d[b] / rr) - (a == b ? 1.0 : 0.0));
         }
 }
@@ -542,16 +542,26 @@ void split_host(const std::vector<double> &x, std::vector<__bf16> &hi, std::vect
     }
 }
 
+// Replica layout sizes: particles padded to a multiple of kParticlePad
+// (ld = 3 Np), replicas to a multiple of the 128-row block tile.
+int replica_ld(int64_t n) { return 3 * pad_to(n, mp::kParticlePad); }
+int replica_rows(int64_t reps) { return pad_to(reps, mp::kBM); }
+
 void launch_product(int mode, const float *a32, const __bf16 *a_hi, const __bf16 *a_lo,
                     const float *b32, const __bf16 *b_hi, const __bf16 *b_lo, float *c, int rows,
                     int ld, hipStream_t s)
 {
-    const dim3 grid((unsigned)(rows / mp::kGemmTile), (unsigned)(ld / mp::kGemmTile));
-    if (mode == MP_DIPOLAR_GEMM_F32)
+    require(rows % mp::kBM == 0 && ld % mp::kBN == 0 && ld % mp::kGemmTile == 0 &&
+                ld % (6 * mp::kBK) == 0,
+            "replica product: rows or ld do not fit the tiles");
+    if (mode == MP_DIPOLAR_GEMM_F32) {
+        const dim3 grid((unsigned)(rows / mp::kGemmTile), (unsigned)(ld / mp::kGemmTile));
         hipLaunchKernelGGL(mp::gemm_f32_kernel, grid, dim3(256), 0, s, a32, b32, c, ld, ld);
-    else
-        hipLaunchKernelGGL(mp::gemm_bf16x3_kernel, grid, dim3(256), 0, s, a_hi, a_lo, b_hi, b_lo,
-                           c, ld, ld);
+    } else {
+        const int nrb = rows / mp::kBM, ncb = ld / mp::kBN;
+        hipLaunchKernelGGL(mp::gemm_bf16x3_v2_kernel, dim3((unsigned)(nrb * ncb)), dim3(256), 0,
+                           s, a_hi, a_lo, b_hi, b_lo, c, ld, ld, nrb, ncb);
+    }
 }
 
 // All members are one cluster replicated: state [rows][ld] replica-major,
@@ -576,8 +586,8 @@ struct ReplicaRun : GroupBase {
         mode = m;
         reps = (int)sp.n_members;
         n = (int)(sp.member_offsets[1] - sp.member_offsets[0]);
-        ld = pad_to(3 * (int64_t)n, mp::kGemmTile);
-        rows = pad_to(reps, mp::kGemmTile);
+        ld = replica_ld(n);
+        rows = replica_rows(reps);
         sched = plan.sched[0];
         S = (int64_t)sched.size();
         n_total = plan.group_steps[0];
@@ -636,8 +646,10 @@ struct ReplicaRun : GroupBase {
         std::vector<double> m0((size_t)rows * ld, 0.0);
         const double *src = sp.start_state ? sp.start_state : sp.magnetisation_direction;
         for (int rr = 0; rr < reps; ++rr)
-            for (int c = 0; c < 3 * n; ++c)
-                m0[(size_t)rr * ld + c] = src[3 * sp.member_offsets[rr] + c];
+            for (int p = 0; p < n; ++p)
+                for (int c = 0; c < 3; ++c)
+                    m0[(size_t)rr * ld + mp::replica_col(p, c)] =
+                        src[3 * (sp.member_offsets[rr] + p) + c];
         n_done = std::min(sp.start_step, n_total);
         d_m.upload(cast_vec<float>(m0));
         d_mt.upload(std::vector<float>(m0.size(), 0.f));
@@ -715,8 +727,9 @@ struct ReplicaRun : GroupBase {
         MP_HIP(hipMemcpy(hm.data(), d_m.p, hm.size() * sizeof(float), hipMemcpyDeviceToHost));
         out.resize((size_t)reps * n * 3);
         for (int rr = 0; rr < reps; ++rr)
-            for (int c = 0; c < 3 * n; ++c)
-                out[(size_t)rr * n * 3 + c] = hm[(size_t)rr * ld + c];
+            for (int p = 0; p < n; ++p)
+                for (int c = 0; c < 3; ++c)
+                    out[((size_t)rr * n + p) * 3 + c] = hm[(size_t)rr * ld + mp::replica_col(p, c)];
     }
 
     GroupOut collect(hipStream_t s) override
@@ -1029,8 +1042,8 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
     // interacting clusters: one block holds whole clusters, so the block must
     // be at least the largest cluster
     int cluster_block = 0;
+    int64_t nmax = 1;
     if (sp.interactions) {
-        int64_t nmax = 1;
         for (int64_t m = 0; m < sp.n_members; ++m)
             nmax = std::max(nmax, sp.member_offsets[m + 1] - sp.member_offsets[m]);
         if (nmax > 1) {
@@ -1038,9 +1051,6 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
             if (cluster_block == 0)
                 throw simulation::Error(MP_ERR_UNSUPPORTED,
                                         "clusters of more than 1024 particles are not supported");
-            const bool fixed = std::getenv("MAGPY_AMD_CLUSTER_BLOCK") != nullptr;
-            if (!fixed && sp.noise == MP_NOISE_PHILOX && plan.group_steps[0] >= 20000)
-                cluster_block = autotune_cluster_block(sp, plan, nmax, ss->sg.s);
         }
     }
     // replicas of one cluster: the dipolar field as one product per stage
@@ -1064,6 +1074,10 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
         throw simulation::Error(MP_ERR_UNSUPPORTED,
                                 "the dipolar product needs an FP32 Heun ensemble of replicas of "
                                 "one cluster with the Philox stream");
+    // pair sums: pick the block size by timing the candidates on long runs
+    if (cluster_block != 0 && std::getenv("MAGPY_AMD_CLUSTER_BLOCK") == nullptr &&
+        sp.noise == MP_NOISE_PHILOX && plan.group_steps[0] >= 20000)
+        cluster_block = autotune_cluster_block(sp, plan, nmax, ss->sg.s);
     ss->dipolar = cluster_block ? MP_DIPOLAR_PAIRS : 0;
     ss->cluster_block = cluster_block;
     for (int g = 0; g < G; ++g) {
@@ -1348,13 +1362,15 @@ int mp_dipolar_field(const double *location, const double *radius, const double 
         MP_HIP(hipSetDevice(device));
         const simulation::ReducedMember r = simulation::reduce_member(
             radius, anisotropy, n_particles, magnetisation, 0.1, 0.0, 1.0, 1.0, 0.0, 0.0);
-        const int ld = pad_to(3 * (int64_t)n_particles, mp::kGemmTile);
-        const int rows = pad_to((int64_t)reps, mp::kGemmTile);
+        const int ld = replica_ld((int64_t)n_particles);
+        const int rows = replica_rows((int64_t)reps);
         std::vector<double> D;
         dipole_tensor(r, location, magnetisation, n_particles, ld, D);
         std::vector<double> ma((size_t)rows * ld, 0.0);
         for (size_t rr = 0; rr < reps; ++rr)
-            for (size_t c = 0; c < 3 * n_particles; ++c) ma[rr * ld + c] = m[rr * 3 * n_particles + c];
+            for (size_t p = 0; p < n_particles; ++p)
+                for (int c = 0; c < 3; ++c)
+                    ma[rr * ld + mp::replica_col((int)p, c)] = m[(rr * n_particles + p) * 3 + c];
         std::vector<double> out((size_t)rows * ld);
         if (mode == MP_DIPOLAR_PAIRS) {
             DevBuf<double> da, db, dc((size_t)rows * ld);
@@ -1389,7 +1405,9 @@ int mp_dipolar_field(const double *location, const double *radius, const double 
             out.assign(o32.begin(), o32.end());
         }
         for (size_t rr = 0; rr < reps; ++rr)
-            for (size_t c = 0; c < 3 * n_particles; ++c) h[rr * 3 * n_particles + c] = out[rr * ld + c];
+            for (size_t p = 0; p < n_particles; ++p)
+                for (int c = 0; c < 3; ++c)
+                    h[(rr * n_particles + p) * 3 + c] = out[rr * ld + mp::replica_col((int)p, c)];
     });
 }
 

@@ -11,6 +11,14 @@
 // reads of row-major matrices, and a wave's output lanes write consecutive
 // columns of Ht.
 //
+// Columns are laid out component-blocked per 32 particles: particle i,
+// component a sits in column 96 (i / 32) + 32 a + i % 32 (replica_col), with
+// the particle count padded to a multiple of 128 so ld = 3 Np is a multiple
+// of 384.  The same permutation is applied to both indices of D, so the
+// product is unchanged; a 32-column MFMA tile then holds one component of 32
+// consecutive particles, and stage kernels read x, y, z as three coalesced
+// 128-byte rows.
+//
 // The product runs on MFMA in BF16 with the three-term split
 // x = x_hi + x_lo (x_hi = bf16(x), x_lo = bf16(x - x_hi)):
 //   Mt D^T ~ Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T, FP32 accumulation,
@@ -18,6 +26,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "sllg_kernels.hpp"
 
@@ -27,7 +37,10 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 
-constexpr int kGemmTile = 128;  // block tile (rows of Mt) x (rows of D); 4 waves of 64 x 64
+constexpr int kGemmTile = 128;  // block tile of the FP32 product: 4 waves of 64 x 64
+constexpr int kParticlePad = 128;  // particles per replica padded to a multiple of this
+
+__host__ __device__ inline int replica_col(int i, int a) { return 96 * (i >> 5) + 32 * a + (i & 31); }
 
 __device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo)
 {
@@ -48,59 +61,143 @@ __device__ inline void store_tile32(float *C, int ldc, int r0, int c0, const f32
     }
 }
 
-// Ht = Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T.  A = Mt [rows][K],
-// B = D [cols][K], both row-major BF16; C [rows][ldc] FP32.  rows, cols are
-// multiples of kGemmTile and K of 16 (the host pads with zeros).
-static __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(const __bf16 *__restrict__ a_hi,
-                                                          const __bf16 *__restrict__ a_lo,
-                                                          const __bf16 *__restrict__ b_hi,
-                                                          const __bf16 *__restrict__ b_lo,
-                                                          float *__restrict__ c, int K, int ldc)
+// Ht = Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T with LDS staging.  Block
+// tile 128 rows x 192 columns (4 waves of 64 x 96 = 2 x 3 MFMA tiles), k in
+// stages of 32 double-buffered in LDS: while the waves run a stage's 36
+// MFMAs each, the block's next stage (A hi/lo 128 x 32, B hi/lo 192 x 32) is
+// already in flight from global memory, three stages ahead in registers.
+// Blocks are numbered so that the blocks one XCD runs (block ids b with
+// equal b % 8) share column panels of D, which then stay in that XCD's L2.
+// rows % 128 == 0, cols % 192 == 0, K % 192 == 0 (stages come in sixes).
+constexpr int kBM = 128, kBN = 192, kBK = 32, kLdsPad = 8;
+constexpr int kLdsRow = kBK + kLdsPad;  // bf16 per LDS row
+
+__global__ __launch_bounds__(256) void gemm_bf16x3_v2_kernel(const __bf16 *__restrict__ a_hi,
+                                                             const __bf16 *__restrict__ a_lo,
+                                                             const __bf16 *__restrict__ b_hi,
+                                                             const __bf16 *__restrict__ b_lo,
+                                                             float *__restrict__ c, int K, int ldc,
+                                                             int n_row_blocks, int n_col_blocks)
 {
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int r0 = blockIdx.x * kGemmTile + (wave >> 1) * 64;
-    const int c0 = blockIdx.y * kGemmTile + (wave & 1) * 64;
-    const int lr = l & 31, kq = 8 * (l >> 5);
-    f32x16 acc[2][2];
+    __shared__ __attribute__((aligned(16))) __bf16 sa[2][2][kBM * kLdsRow];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[2][2][kBN * kLdsRow];
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    // XCD-aware block order: consecutive logical blocks share a column block
+    const int total = n_row_blocks * n_col_blocks;
+    int logical = blockIdx.x;
+    if (total % 8 == 0) logical = (blockIdx.x % 8) * (total / 8) + blockIdx.x / 8;
+    const int cb = logical / n_row_blocks, rb = logical % n_row_blocks;
+    const int row0 = rb * kBM, col0 = cb * kBN;
+    const int wr = wave >> 1, wc = wave & 1;
+    f32x16 acc[2][3];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < 3; ++u)
 #pragma unroll
             for (int q = 0; q < 16; ++q) acc[t][u][q] = 0.f;
-    const bf16x8 *ah[2], *al[2], *bh[2], *bl[2];
+
+    // global -> register staging, three stages ahead: A panel 128 rows x 4
+    // chunks of 8 bf16 (2 chunks per thread per part), B panel 192 rows x 4
+    // chunks (3 per thread per part).  Register set (s % 3) holds stage s.
+    typedef __attribute__((ext_vector_type(4))) uint32_t u32x4v;
+    u32x4v ga[3][2][2], gb[3][2][3];
+    const int n_stages = K / kBK;
+    auto load_stage = [&](auto set_c, int st) {
+        constexpr int R = decltype(set_c)::value;
+        if (st >= n_stages) return;
+        const int k0 = st * kBK;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const size_t ra = (size_t)(r0 + 32 * t + lr) * K + kq;
-        const size_t rb = (size_t)(c0 + 32 * t + lr) * K + kq;
-        ah[t] = reinterpret_cast<const bf16x8 *>(a_hi + ra);
-        al[t] = reinterpret_cast<const bf16x8 *>(a_lo + ra);
-        bh[t] = reinterpret_cast<const bf16x8 *>(b_hi + rb);
-        bl[t] = reinterpret_cast<const bf16x8 *>(b_lo + rb);
-    }
-    for (int k = 0; k < K; k += 16) {
-        const int o = k / 8;  // bf16x8 elements
-        bf16x8 xa_h[2], xa_l[2], xb_h[2], xb_l[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            xa_h[t] = ah[t][o];
-            xa_l[t] = al[t][o];
-            xb_h[t] = bh[t][o];
-            xb_l[t] = bl[t][o];
+        for (int i = 0; i < 2; ++i) {
+            const int ch = tid + 256 * i, row = ch >> 2, kc = ch & 3;
+            const size_t off = (size_t)(row0 + row) * K + k0 + 8 * kc;
+            ga[R][0][i] = *reinterpret_cast<const u32x4v *>(a_hi + off);
+            ga[R][1][i] = *reinterpret_cast<const u32x4v *>(a_lo + off);
         }
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+        for (int i = 0; i < 3; ++i) {
+            const int ch = tid + 256 * i, row = ch >> 2, kc = ch & 3;
+            const size_t off = (size_t)(col0 + row) * K + k0 + 8 * kc;
+            gb[R][0][i] = *reinterpret_cast<const u32x4v *>(b_hi + off);
+            gb[R][1][i] = *reinterpret_cast<const u32x4v *>(b_lo + off);
+        }
+    };
+    auto store_stage = [&](auto set_c, auto buf_c) {
+        constexpr int R = decltype(set_c)::value, B = decltype(buf_c)::value;
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa_h[t], xb_h[u], acc[t][u], 0, 0, 0);
-                acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa_h[t], xb_l[u], acc[t][u], 0, 0, 0);
-                acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa_l[t], xb_h[u], acc[t][u], 0, 0, 0);
-            }
+        for (int i = 0; i < 2; ++i) {
+            const int ch = tid + 256 * i, row = ch >> 2, kc = ch & 3;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                *reinterpret_cast<u32x4v *>(&sa[B][h][row * kLdsRow + 8 * kc]) = ga[R][h][i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int ch = tid + 256 * i, row = ch >> 2, kc = ch & 3;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                *reinterpret_cast<u32x4v *>(&sb[B][h][row * kLdsRow + 8 * kc]) = gb[R][h][i];
+        }
+    };
+    auto compute = [&](auto buf_c) {
+        constexpr int B = decltype(buf_c)::value;
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 16) {
+            const int kq = kk + 8 * (l >> 5);
+            bf16x8 xa[2][2], xb[2][3];
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    xa[h][t] = *reinterpret_cast<const bf16x8 *>(
+                        &sa[B][h][(64 * wr + 32 * t + (l & 31)) * kLdsRow + kq]);
+#pragma unroll
+            for (int u = 0; u < 3; ++u)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    xb[h][u] = *reinterpret_cast<const bf16x8 *>(
+                        &sb[B][h][(96 * wc + 32 * u + (l & 31)) * kLdsRow + kq]);
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[0][t], xb[0][u], acc[t][u], 0, 0, 0);
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[0][t], xb[1][u], acc[t][u], 0, 0, 0);
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[1][t], xb[0][u], acc[t][u], 0, 0, 0);
+                }
+        }
+    };
+    // stage s: compute from LDS buffer s % 2, then move stage s + 1 from
+    // register set (s + 1) % 3 into the other buffer and refill that set
+    // with stage s + 4.
+    auto step = [&](auto s_c, int st) {
+        constexpr int S6 = decltype(s_c)::value;   // st % 6
+        compute(std::integral_constant<int, S6 % 2>{});
+        if (st + 1 < n_stages)
+            store_stage(std::integral_constant<int, (S6 + 1) % 3>{},
+                        std::integral_constant<int, (S6 + 1) % 2>{});
+        __syncthreads();
+        load_stage(std::integral_constant<int, (S6 + 1) % 3>{}, st + 4);
+    };
+    load_stage(std::integral_constant<int, 0>{}, 0);
+    store_stage(std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{});
+    __syncthreads();
+    load_stage(std::integral_constant<int, 1>{}, 1);
+    load_stage(std::integral_constant<int, 2>{}, 2);
+    load_stage(std::integral_constant<int, 0>{}, 3);
+    for (int st = 0; st < n_stages; st += 6) {
+        step(std::integral_constant<int, 0>{}, st);
+        step(std::integral_constant<int, 1>{}, st + 1);
+        step(std::integral_constant<int, 2>{}, st + 2);
+        step(std::integral_constant<int, 3>{}, st + 3);
+        step(std::integral_constant<int, 4>{}, st + 4);
+        step(std::integral_constant<int, 5>{}, st + 5);
     }
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) store_tile32(c, ldc, r0 + 32 * t, c0 + 32 * u, acc[t][u]);
+        for (int u = 0; u < 3; ++u)
+            store_tile32(c, ldc, row0 + 64 * wr + 32 * t, col0 + 96 * wc + 32 * u, acc[t][u]);
 }
 
 // The same product in FP32 on MFMA (v_mfma_f32_32x32x2f32).  Each lane reads
@@ -181,7 +278,7 @@ static __global__ void split_kernel(const float *__restrict__ x, __bf16 *__restr
 struct ReplicaArgs {
     int n;                  // particles per replica
     int reps;               // replicas
-    int ld;                 // padded row length (>= 3n)
+    int ld;                 // row length 3 Np (Np = n padded to a multiple of 128)
     int n_waves;            // waves in the grid (ensemble partial stride)
     const float *kex, *key, *kez, *edx, *edy, *edz;  // per particle: k e, e dt
     const float *sig;       // per particle: sr sqrt(dt)
@@ -230,9 +327,9 @@ static __global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n * a.reps) return;
     const int r = i / a.n, p = i % a.n;
-    const size_t o = (size_t)r * a.ld + 3 * p;
-    const float mx = a.m[o], my = a.m[o + 1], mz = a.m[o + 2];
-    const float hx = a.h[o], hy = a.h[o + 1], hz = a.h[o + 2];
+    const size_t ox = (size_t)r * a.ld + replica_col(p, 0), oy = ox + 32, oz = ox + 64;
+    const float mx = a.m[ox], my = a.m[oy], mz = a.m[oz];
+    const float hx = a.h[ox], hy = a.h[oy], hz = a.h[oz];
     float sx, sy, sz;
     replica_normals(a, r, p, n_step, a.sig[p], sx, sy, sz);
     const float dt = a.dt;
@@ -242,11 +339,11 @@ static __global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs 
     float ax, ay, az;
     llg_map(mx, my, mz, vx, vy, vz, a.alpha, ax, ay, az);
     const float tx = mx + ax, ty = my + ay, tz = mz + az;
-    a.delta[o] = ax; a.delta[o + 1] = ay; a.delta[o + 2] = az;
-    a.mt[o] = tx; a.mt[o + 1] = ty; a.mt[o + 2] = tz;
-    split_bf16(tx, a.m_hi[o], a.m_lo[o]);
-    split_bf16(ty, a.m_hi[o + 1], a.m_lo[o + 1]);
-    split_bf16(tz, a.m_hi[o + 2], a.m_lo[o + 2]);
+    a.delta[ox] = ax; a.delta[oy] = ay; a.delta[oz] = az;
+    a.mt[ox] = tx; a.mt[oy] = ty; a.mt[oz] = tz;
+    split_bf16(tx, a.m_hi[ox], a.m_lo[ox]);
+    split_bf16(ty, a.m_hi[oy], a.m_lo[oy]);
+    split_bf16(tz, a.m_hi[oz], a.m_lo[oz]);
 }
 
 // stage 2: m' = m + (L(m, v1) + L(m~, v2)) / 2, stores m', its split, samples
@@ -257,9 +354,9 @@ static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs 
     const bool live = i < a.n * a.reps;
     const int q = live ? i : a.n * a.reps - 1;
     const int r = q / a.n, p = q % a.n;
-    const size_t o = (size_t)r * a.ld + 3 * p;
-    const float tx = a.mt[o], ty = a.mt[o + 1], tz = a.mt[o + 2];
-    const float hx = a.h[o], hy = a.h[o + 1], hz = a.h[o + 2];
+    const size_t ox = (size_t)r * a.ld + replica_col(p, 0), oy = ox + 32, oz = ox + 64;
+    const float tx = a.mt[ox], ty = a.mt[oy], tz = a.mt[oz];
+    const float hx = a.h[ox], hy = a.h[oy], hz = a.h[oz];
     float sx, sy, sz;
     replica_normals(a, r, p, n_step, a.sig[p], sx, sy, sz);
     const float dt = a.dt;
@@ -268,9 +365,9 @@ static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs 
                 vz = d * a.edz[p] + ((hz + h2) * dt + sz);
     float bx, by, bz;
     llg_map(tx, ty, tz, vx, vy, vz, a.alpha, bx, by, bz);
-    float mx = a.m[o] + 0.5f * (a.delta[o] + bx);
-    float my = a.m[o + 1] + 0.5f * (a.delta[o + 1] + by);
-    float mz = a.m[o + 2] + 0.5f * (a.delta[o + 2] + bz);
+    float mx = a.m[ox] + 0.5f * (a.delta[ox] + bx);
+    float my = a.m[oy] + 0.5f * (a.delta[oy] + by);
+    float mz = a.m[oz] + 0.5f * (a.delta[oz] + bz);
     if (a.renorm) {
         const float nrm = sqrtf(mx * mx + my * my + mz * mz);
         mx = mx / nrm;
@@ -278,10 +375,10 @@ static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs 
         mz = mz / nrm;
     }
     if (live) {
-        a.m[o] = mx; a.m[o + 1] = my; a.m[o + 2] = mz;
-        split_bf16(mx, a.m_hi[o], a.m_lo[o]);
-        split_bf16(my, a.m_hi[o + 1], a.m_lo[o + 1]);
-        split_bf16(mz, a.m_hi[o + 2], a.m_lo[o + 2]);
+        a.m[ox] = mx; a.m[oy] = my; a.m[oz] = mz;
+        split_bf16(mx, a.m_hi[ox], a.m_lo[ox]);
+        split_bf16(my, a.m_hi[oy], a.m_lo[oy]);
+        split_bf16(mz, a.m_hi[oz], a.m_lo[oz]);
         if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[i] |= 1u;
     }
     // samples s_first .. s_first + n_rec - 1 are all recorded after this step
