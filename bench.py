@@ -251,21 +251,22 @@ def sar_run(n, precision, save, stop=0, checkpoint="", resume="", noise_tag=0):
 
 # Arithmetic of the sensitivity kernels per particle-step, counted from
 # adjoint_kernels.hpp (add/sub/mul = 1, the FP64 gradient sums included):
-# one forward Heun step of the fused form is 118 FLOP; the reverse pass
-# recomputes the step (118) and runs its adjoint (325).
+# one forward Heun step of the fused form is 118 FLOP; the transfer kernel
+# recomputes the step (118), its intermediates (75) and runs three adjoint
+# channels (3 x 250).
 FLOP_ADJ_FORWARD = 118
-FLOP_ADJ_REVERSE = 118 + 325
+FLOP_ADJ_TRANSFER = 118 + 75 + 3 * 250
 
 
 def adjoint_bench(args):
     """Gradient throughput of the differentiation workload: 16,384 particles
     of the AC-field distribution, theta = (K, alpha, H, median r) at
     (23 kJ/m^3, 0.1, 20 mT, 10 nm), checkpoints every 1,000 steps.  A bench
-    step is one block of time steps; the run is warmup + steps blocks long,
-    its forward pass is timed block by block and its reverse pass likewise
-    (the reverse's first `steps` blocks, which mirror the timed forward
-    blocks).  value = particle-steps / (forward + reverse wall time) of the
-    timed blocks."""
+    step is one block of time steps through the recording forward pass (the
+    forward kernel plus the adjoint maps of the block's segments); the run is
+    warmup + steps blocks long and ends with the reverse pass (a scan over the
+    sampling intervals, timed and added).  value = particle-steps of the timed
+    blocks / (their wall time + the reverse pass)."""
     import magpy_amd
     core = magpy_amd.core
     P = args.particles or DEFAULTS["adjoint"][0]
@@ -280,31 +281,27 @@ def adjoint_bench(args):
     print("device:", core.device_info(0), flush=True)
     os.environ["MAGPY_AMD_ADJOINT_STEPS_PER_LAUNCH"] = str(B)
     run = core.Adjoint(shape, ax, ax, inp["seeds"], 400e3, 300.0, 3e5, 1e-14, N, ks, 1000, "fp32")
-    run.begin(theta)
+    run.begin(theta, True)
     t_fwd = 0.0
     for b in range(n_blocks):
+        if b == args.warmup:
+            info0 = run.info()
         t0 = time.perf_counter()
         run.forward_advance(B)
         if b >= args.warmup:
             t_fwd += time.perf_counter() - t0
+    info1 = run.info()
     ens, fin, status = run.forward_result()
     w = np.linspace(-1.0, 1.0, len(ks))
-    run.reverse_begin(w, 1.0)
-    t_rev = 0.0
-    left = N
-    rev_blocks = 0
-    while left > 0:
-        t0 = time.perf_counter()
-        left = run.reverse_advance(B)
-        rev_blocks += 1
-        if rev_blocks <= args.steps:
-            t_rev += time.perf_counter() - t0
-    grad = run.reverse_result()
-    info = run.info()
+    t0 = time.perf_counter()
+    grad = run.reverse(w, 1.0)
+    t_rev = time.perf_counter() - t0
+    info2 = run.info()
     ps = float(P) * B * args.steps
     value = ps / (t_fwd + t_rev)
-    rev_rate = ps / t_rev
-    achieved = FLOP_ADJ_REVERSE * rev_rate / 1e12
+    fwd_kernel_s = (info1["forward_ms"] - info0["forward_ms"]) / 1e3
+    rec_kernel_s = (info1["reverse_ms"] - info0["reverse_ms"]) / 1e3
+    achieved = FLOP_ADJ_TRANSFER * ps / rec_kernel_s / 1e12
     line = {
         "metric": METRIC, "value": value, "unit": "gradient particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -312,19 +309,19 @@ def adjoint_bench(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "differentiation: 16,384 AC-field particles (20 mT, 300 kHz, "
                                "dt=1e-14 s), pathwise Heun adjoint, FP32 state, checkpoints "
-                               "every 1,000 steps; step = %d time steps forward + reverse" % B,
+                               "every 1,000 steps; step = %d time steps of the recording "
+                               "forward pass" % B,
                    "particles": P, "time_steps_per_step": B, "run_length_steps": N,
-                   "checkpoint_bytes": info["checkpoint_bytes"],
-                   "scratch_bytes": info["scratch_bytes"], "forward_status": int(status),
+                   "samples": len(ks), "checkpoint_bytes": info2["checkpoint_bytes"],
+                   "scratch_bytes": info2["scratch_bytes"], "forward_status": int(status),
                    "gradient_finite": bool(np.all(np.isfinite(grad)))},
-        "forward_particle_steps_per_s": ps / t_fwd, "reverse_particle_steps_per_s": rev_rate,
-        "roofline": {"bound": "valu", "kernel": "path_reverse_kernel<float>",
+        "reverse_pass_ms": t_rev * 1e3,
+        "device_s_forward_kernels": fwd_kernel_s, "device_s_adjoint_maps": rec_kernel_s,
+        "roofline": {"bound": "valu", "kernel": "path_transfer_kernel<float> (+ folding)",
                      "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None,
-                     "flop_per_particle_step": FLOP_ADJ_REVERSE,
-                     "forward_achieved": FLOP_ADJ_FORWARD * ps / t_fwd / 1e12,
-                     "device_ms_forward_all_blocks": info["forward_ms"],
-                     "device_ms_reverse_all_blocks": info["reverse_ms"]},
+                     "flop_per_particle_step": FLOP_ADJ_TRANSFER,
+                     "forward_kernel_achieved": FLOP_ADJ_FORWARD * ps / fwd_kernel_s / 1e12},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = adjoint_cpu_baseline(inp, theta)

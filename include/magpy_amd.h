@@ -212,8 +212,11 @@ int mp_wiener_normals(uint64_t seed, uint32_t particle, uint32_t tag, uint64_t j
  * after n_steps.  A reverse pass returns the gradient of
  *   J = sum_s grad_ens_mz[s] * ens_mz[s] + grad_final_mz * final_mz
  * with respect to theta through the trajectories (the vector-Jacobian
- * product): the forward states are kept every checkpoint_interval steps in
- * device memory and recomputed in between.  No counterpart in the reference,
+ * product).  While a recording forward pass runs, the adjoint's linear maps
+ * between sample steps are computed from states kept every
+ * checkpoint_interval steps (recomputed in between) and held per sampling
+ * interval in device memory (21 doubles per particle per interval), so the
+ * reverse pass for any weights is a short scan.  No counterpart in the reference,
  * which has no derivatives; magpy_amd.adjoint wraps these calls in a
  * torch.autograd.Function. */
 typedef struct mp_adjoint_spec {
@@ -238,8 +241,9 @@ typedef struct mp_adjoint_spec {
 typedef struct mp_adjoint mp_adjoint;
 int mp_adjoint_create(const mp_adjoint_spec *spec, mp_adjoint **out);
 void mp_adjoint_destroy(mp_adjoint *a);
-/* start a forward pass at theta (drops any earlier pass) */
-int mp_adjoint_begin(mp_adjoint *a, const double theta[4]);
+/* start a forward pass at theta (drops any earlier pass); record_gradient
+ * = 0 runs the forward pass only (values, e.g. for finite differences) */
+int mp_adjoint_begin(mp_adjoint *a, const double theta[4], int record_gradient);
 /* advance the forward pass by up to max_steps; *steps_done = steps so far */
 int mp_adjoint_forward_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_done);
 /* finish the forward pass and return its outputs (ens_mz [S], final_mz);
@@ -247,12 +251,14 @@ int mp_adjoint_forward_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_
 int mp_adjoint_forward_result(mp_adjoint *a, double *ens_mz, double *final_mz, uint32_t *status);
 /* start a reverse pass of the finished forward pass for the given weights */
 int mp_adjoint_reverse_begin(mp_adjoint *a, const double *grad_ens_mz, double grad_final_mz);
-/* advance the reverse pass by up to max_steps; *steps_left = steps still to go */
+/* run the reverse pass (max_steps is ignored: the scan is one launch);
+ * *steps_left = 0 */
 int mp_adjoint_reverse_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_left);
 /* finish the reverse pass: grad_theta[4] = dJ/d(K, alpha, H, r) */
 int mp_adjoint_reverse_result(mp_adjoint *a, double grad_theta[4]);
-/* device milliseconds of the forward and reverse kernels so far and the
- * device bytes held for checkpoints and segment scratch */
+/* device milliseconds of the forward kernels and of the gradient kernels
+ * (transfers, folding, scan) so far, and the device bytes held for
+ * checkpoints and segment scratch */
 int mp_adjoint_info(mp_adjoint *a, double *forward_ms, double *reverse_ms, int64_t *checkpoint_bytes,
                     int64_t *scratch_bytes);
 /* Reduced per-particle parameters q[p] = (dt, sr sqrt(dt), alpha, happ) at

@@ -3,9 +3,10 @@
 `EnsembleTrajectory` is a torch.autograd.Function around the device passes
 of `core.Adjoint` (include/magpy_amd.h, mp_adjoint_*): its forward returns
 the sampled ensemble magnetisation <M_z>(t_s) (A/m) and the final mean
-reduced m_z of an ensemble of independent particles, and its backward runs
-the hand-written adjoint of the Heun step over the same Wiener increments,
-from checkpoints, for the incoming gradients.  `SARObjective` builds the
+reduced m_z of an ensemble of independent particles, recording (when theta
+needs a gradient) the hand-written adjoint of the Heun step over the same
+Wiener increments as linear maps between sample steps; its backward applies
+them to the incoming gradients.  `SARObjective` builds the
 workload (one lognormal ensemble, sine field, the reference's sampling) and
 evaluates the hysteresis energy the way magpy/results.py:167-217 does,
 E = -mu0 * trapz(H, <M_z>) over the last field period, SAR = f * E, in torch
@@ -33,7 +34,9 @@ class EnsembleTrajectory(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, theta, runner):
-        ens, fin, status = runner.forward(theta.detach().cpu().to(torch.float64).numpy())
+        # the gradient is recorded only when theta needs one
+        ens, fin, status = runner.forward(theta.detach().cpu().to(torch.float64).numpy(),
+                                          record_gradient=bool(ctx.needs_input_grad[0]))
         if status:
             raise FloatingPointError("non-finite state in the forward pass (status %d)" % status)
         ctx.runner = runner
@@ -46,7 +49,7 @@ class EnsembleTrajectory(torch.autograd.Function):
         runner = ctx.runner
         if runner.forward_passes != ctx.pass_id:
             raise RuntimeError("the runner ran another forward pass since this one; "
-                               "the checkpoints of this pass are gone")
+                               "the adjoint maps recorded by this pass are gone")
         g = np.zeros(runner.n_samples) if g_ens is None else \
             g_ens.detach().cpu().to(torch.float64).numpy()
         gf = 0.0 if g_fin is None else float(g_fin)

@@ -100,7 +100,7 @@ cdef extern from "magpy_amd.h" nogil:
         pass
     int mp_adjoint_create(const mp_adjoint_spec *spec, mp_adjoint **out)
     void mp_adjoint_destroy(mp_adjoint *a)
-    int mp_adjoint_begin(mp_adjoint *a, const double *theta)
+    int mp_adjoint_begin(mp_adjoint *a, const double *theta, int record_gradient)
     int mp_adjoint_forward_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_done)
     int mp_adjoint_forward_result(mp_adjoint *a, double *ens_mz, double *final_mz,
                                   uint32_t *status)
@@ -611,7 +611,7 @@ cdef class Adjoint:
     (mp_adjoint_*).  theta = (anisotropy K, damping alpha, field amplitude H,
     median radius).  forward(theta) -> (ens_mz [S], final_mz, status);
     reverse(grad_ens_mz, grad_final_mz) -> dJ/dtheta [4] for the last
-    forward pass.  Both also exist as begin/advance/result steps."""
+    recording forward pass.  Both also exist as begin/advance/result steps."""
     cdef mp_adjoint *a
     cdef _AdjointSpec spec
     cdef readonly int64_t n_particles, n_steps, n_samples
@@ -638,12 +638,14 @@ cdef class Adjoint:
             mp_adjoint_destroy(self.a)
             self.a = NULL
 
-    def begin(self, theta):
+    def begin(self, theta, record_gradient=True):
+        """Start a forward pass at theta; record_gradient=False computes
+        values only (no reverse pass possible)."""
         th = _c(theta, np.float64)
         if th.shape[0] != 4:
             raise ValueError("theta has four entries (K, alpha, H, radius)")
         cdef const double[::1] th_v = th
-        _check(mp_adjoint_begin(self.a, &th_v[0]))
+        _check(mp_adjoint_begin(self.a, &th_v[0], 1 if record_gradient else 0))
         self.forward_passes += 1
 
     def forward_advance(self, int64_t n):
@@ -665,8 +667,8 @@ cdef class Adjoint:
         _check(rc)
         return ens[:self.n_samples], fin, st
 
-    def forward(self, theta):
-        self.begin(theta)
+    def forward(self, theta, record_gradient=True):
+        self.begin(theta, record_gradient)
         return self.forward_result()
 
     def reverse_begin(self, grad_ens_mz, double grad_final_mz):

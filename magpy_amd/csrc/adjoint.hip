@@ -1,8 +1,10 @@
 // adjoint.hip -- host side of the pathwise sensitivities (mp_adjoint_* in
 // include/magpy_amd.h): the SI -> reduced parameters of every particle and
-// their derivatives with respect to theta = (K, alpha, H, median radius),
-// the forward pass with device checkpoints, the reverse pass over segments,
-// and the chain rule from the per-particle reduced gradients to theta.
+// their derivatives with respect to theta = (K, alpha, H, median radius);
+// the forward pass in blocks, each followed (when the gradient is recorded)
+// by the transfers of its segments, computed in parallel and folded into
+// one transfer per sampling interval; the reverse pass as a scan over the
+// intervals; and the chain rule from per-particle reduced gradients to theta.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -100,7 +102,7 @@ int64_t env_steps_per_launch()
 
 struct AdjointBase {
     virtual ~AdjointBase() = default;
-    virtual void begin(const double th[4]) = 0;
+    virtual void begin(const double th[4], bool record) = 0;
     virtual int64_t forward_advance(int64_t n) = 0;
     virtual void forward_result(double *ens, double *fin, uint32_t *status) = 0;
     virtual void reverse_begin(const double *w, double wf) = 0;
@@ -110,24 +112,68 @@ struct AdjointBase {
     int64_t ck_bytes = 0, scratch_bytes = 0;
 };
 
+// Pieces of the run: every segment [jC, min(jC + C, N)) is cut at the
+// sample steps strictly inside it; a piece that starts at a sample step (or
+// at step 0) starts a sampling interval.  Built once per handle.
+struct PieceTable {
+    std::vector<int64_t> bounds;                 // interval starts: 0 and the samples in (0, N)
+    std::vector<int32_t> seg_first, seg_count;   // per segment
+    std::vector<int64_t> start;                  // per piece: first step
+    std::vector<int32_t> is_new, iv;             // per piece
+
+    void build(const std::vector<int64_t> &sched, int64_t N, int64_t C)
+    {
+        bounds.assign(1, 0);
+        for (int64_t k : sched)
+            if (k > 0 && k < N && k != bounds.back()) bounds.push_back(k);
+        const int64_t n_seg = (N + C - 1) / C;
+        seg_first.resize((size_t)n_seg);
+        seg_count.resize((size_t)n_seg);
+        size_t bi = 0;   // next boundary to place
+        for (int64_t j = 0; j < n_seg; ++j) {
+            const int64_t a = j * C, b = std::min(N, a + C);
+            seg_first[(size_t)j] = (int32_t)start.size();
+            while (bi < bounds.size() && bounds[bi] < a) ++bi;
+            auto add = [&](int64_t x) {
+                // interval = last boundary <= x
+                const size_t k = (size_t)(std::upper_bound(bounds.begin(), bounds.end(), x) -
+                                          bounds.begin()) - 1;
+                start.push_back(x);
+                is_new.push_back(bounds[k] == x ? 1 : 0);
+                iv.push_back((int32_t)k);
+            };
+            add(a);
+            size_t bj = bi;
+            while (bj < bounds.size() && bounds[bj] < b) {
+                if (bounds[bj] > a) add(bounds[bj]);
+                ++bj;
+            }
+            seg_count[(size_t)j] = (int32_t)(start.size() - seg_first[(size_t)j]);
+        }
+        require(start.size() < (size_t)INT32_MAX, "too many pieces");
+    }
+};
+
 template <typename T>
 struct AdjointRun final : AdjointBase {
     mp_adjoint_spec sp;
     std::vector<double> shape, m0;
     std::vector<int64_t> sched;
+    PieceTable pt;
     int device;
-    int64_t P, N, C, S, n_ck, spl;
+    int64_t P, N, C, S, n_ck, spl, segs_per_block, batch_segs, max_batch_pieces;
     int n_waves;
     StreamGuard sg;
-    DevBuf<T> ex, ey, ez, sig, mx, my, mz, ck, scratch, ftab, lam;
+    DevBuf<T> ex, ey, ez, sig, mx, my, mz, ck, scratch, ftab;
     DevBuf<uint32_t> key0, key1, status;
-    DevBuf<double> partial, c_sample, g;
-    DevBuf<int64_t> d_sched;
+    DevBuf<double> partial, pieces, acc, intervals, inj, g;
+    DevBuf<int64_t> d_sched, d_piece_start;
+    DevBuf<int32_t> d_seg_first, d_seg_count, d_piece_new, d_piece_iv;
     // pass state
-    double theta[4] = {0, 0, 0, 0};
     std::vector<double> q, dq;
-    bool begun = false, fwd_finished = false, rev_begun = false;
-    int64_t fwd_done = 0, seg_next = -1;
+    bool begun = false, recording = false, fwd_finished = false, rev_begun = false, rev_done = false;
+    int64_t fwd_done = 0;
+    double lambda_z0 = 0.0;
 
     explicit AdjointRun(const mp_adjoint_spec &s) : sp(s), device(s.device)
     {
@@ -137,6 +183,7 @@ struct AdjointRun final : AdjointBase {
         S = sp.n_samples;
         n_ck = (N + C - 1) / C;
         spl = std::max<int64_t>(C, env_steps_per_launch() / C * C);
+        segs_per_block = spl / C;
         shape.assign(sp.radius_shape, sp.radius_shape + P);
         m0.assign(sp.magnetisation_direction, sp.magnetisation_direction + 3 * P);
         sched.assign(sp.sample_steps, sp.sample_steps + S);
@@ -145,7 +192,24 @@ struct AdjointRun final : AdjointBase {
         sp.sample_steps = sched.empty() ? nullptr : sched.data();
         sp.anisotropy_axis = nullptr;   // uploaded below, not kept
         sp.seed = nullptr;
+        pt.build(sched, N, C);
         n_waves = (int)((P + mp::kWave - 1) / mp::kWave);
+        // transfer batches: about 2^18 lanes, scratch at most 4 GiB
+        const int64_t seg_len = std::min(C, N);
+        const int64_t by_lanes = std::max<int64_t>(1, (1 << 18) / P);
+        const int64_t by_bytes =
+            std::max<int64_t>(1, ((int64_t)4 << 30) / (seg_len * 3 * P * (int64_t)sizeof(T)));
+        batch_segs = std::min({segs_per_block, by_lanes, by_bytes});
+        max_batch_pieces = 1;
+        for (int64_t b0 = 0; b0 < n_ck; b0 += segs_per_block) {
+            const int64_t b1 = std::min(n_ck, b0 + segs_per_block);
+            for (int64_t j0 = b0; j0 < b1; j0 += batch_segs) {
+                const int64_t j1 = std::min(b1, j0 + batch_segs);
+                const int64_t np = (j1 < n_ck ? pt.seg_first[(size_t)j1] : (int64_t)pt.start.size()) -
+                                   pt.seg_first[(size_t)j0];
+                max_batch_pieces = std::max(max_batch_pieces, np);
+            }
+        }
         std::vector<T> hx(P), hy(P), hz(P);
         std::vector<uint32_t> k0(P), k1(P);
         for (int64_t p = 0; p < P; ++p) {
@@ -167,17 +231,30 @@ struct AdjointRun final : AdjointBase {
         my.alloc(P);
         mz.alloc(P);
         status.alloc(P);
-        lam.alloc(3 * P);
         g.alloc(4 * P);
-        ck.alloc((size_t)n_ck * 3 * P);
-        scratch.alloc((size_t)std::min(C, N) * 3 * P);
+        ck.alloc((size_t)(segs_per_block + 1) * 3 * P);
         ck_bytes = (int64_t)(ck.n * sizeof(T));
-        scratch_bytes = (int64_t)(scratch.n * sizeof(T));
         partial.alloc((size_t)(S + 1) * n_waves);
-        c_sample.alloc(std::max<int64_t>(S, 1));
         if (S) d_sched.upload(sched);
-        const int64_t tab = std::max(spl, std::min(C, N) * std::max<int64_t>(1, spl / C)) + 2;
-        ftab.alloc((size_t)tab);
+        ftab.alloc((size_t)spl + 2);
+        d_seg_first.upload(pt.seg_first);
+        d_seg_count.upload(pt.seg_count);
+        d_piece_start.upload(pt.start);
+        d_piece_new.upload(pt.is_new);
+        d_piece_iv.upload(pt.iv);
+        inj.alloc(pt.bounds.size());
+    }
+
+    // device buffers of the gradient recording, allocated on first use
+    void alloc_recording()
+    {
+        if (scratch.p) return;
+        const int64_t seg_len = std::min(C, N);
+        scratch.alloc((size_t)seg_len * 3 * (size_t)(batch_segs * P));
+        scratch_bytes = (int64_t)(scratch.n * sizeof(T));
+        pieces.alloc((size_t)max_batch_pieces * mp::kTransfer * P);
+        acc.alloc((size_t)mp::kTransfer * P);
+        intervals.alloc(pt.bounds.size() * mp::kTransfer * (size_t)P);
     }
 
     mp::PathArgs<T> args() const
@@ -201,17 +278,11 @@ struct AdjointRun final : AdjointBase {
         a.n_steps = N;
         a.ck_every = C;
         a.ck = ck.p;
-        a.scratch = scratch.p;
         a.status = status.p;
         a.sched = d_sched.p;
         a.S = S;
         a.mz_partial = partial.p;
         a.n_waves = n_waves;
-        a.c_sample = c_sample.p;
-        a.lx = lam.p;
-        a.ly = lam.p + P;
-        a.lz = lam.p + 2 * P;
-        a.g = g.p;
         return a;
     }
 
@@ -236,10 +307,9 @@ struct AdjointRun final : AdjointBase {
         return ms;
     }
 
-    void begin(const double th[4]) override
+    void begin(const double th[4], bool record) override
     {
         reduce_params(sp, th, q, dq);
-        std::copy(th, th + 4, theta);
         MP_HIP(hipSetDevice(device));
         std::vector<T> hs(P), hx(P), hy(P), hz(P);
         for (int64_t p = 0; p < P; ++p) {
@@ -254,10 +324,65 @@ struct AdjointRun final : AdjointBase {
         MP_HIP(hipMemcpy(mz.p, hz.data(), P * sizeof(T), hipMemcpyHostToDevice));
         MP_HIP(hipMemset(status.p, 0, P * sizeof(uint32_t)));
         MP_HIP(hipMemset(partial.p, 0, partial.n * sizeof(double)));
+        recording = record;
+        if (recording) alloc_recording();
         begun = true;
         fwd_finished = false;
         rev_begun = false;
+        rev_done = false;
         fwd_done = 0;
+    }
+
+    // transfers of segments [j0, j1) (all inside the current forward block,
+    // whose first segment is ck_base), folded into the intervals
+    void record_batch(int64_t j0, int64_t j1, int64_t ck_base)
+    {
+        const int64_t n_a = j0 * C, n_end = std::min(N, j1 * C);
+        table(n_a, n_end - n_a + 2);
+        const int64_t piece0 = pt.seg_first[(size_t)j0];
+        const int64_t piece1 = j1 < n_ck ? pt.seg_first[(size_t)j1] : (int64_t)pt.start.size();
+        mp::TransferArgs<T> t{};
+        t.n = (int)P;
+        t.ex = ex.p;
+        t.ey = ey.p;
+        t.ez = ez.p;
+        t.sig = sig.p;
+        t.key0 = key0.p;
+        t.key1 = key1.p;
+        t.tag = sp.noise_tag;
+        t.dt = (T)q[0];
+        t.alpha = (T)q[2];
+        t.happ = (T)q[3];
+        t.ftab = ftab.p;
+        t.tab0 = n_a;
+        t.n_steps = N;
+        t.ck_every = C;
+        t.ck = ck.p + (size_t)(j0 - ck_base) * 3 * P;
+        t.seg0 = j0;
+        t.n_segs = j1 - j0;
+        t.scratch = scratch.p;
+        t.seg_first_piece = d_seg_first.p + j0;
+        t.seg_n_pieces = d_seg_count.p + j0;
+        t.piece_start = d_piece_start.p;
+        t.piece_base = piece0;
+        t.pieces = pieces.p;
+        const int64_t lanes = (j1 - j0) * P;
+        hipLaunchKernelGGL(mp::path_transfer_kernel<T>, dim3((unsigned)((lanes + 63) / 64)),
+                           dim3(mp::kWave), 0, sg.s, t);
+        MP_HIP(hipGetLastError());
+        mp::ComposeArgs c{};
+        c.n = (int)P;
+        c.n_pieces = (int)(piece1 - piece0);
+        c.pieces = pieces.p;
+        c.piece_new = d_piece_new.p + piece0;
+        c.piece_iv = d_piece_iv.p + piece0;
+        c.acc = acc.p;
+        c.intervals = intervals.p;
+        c.close_last = j1 == n_ck ? 1 : 0;
+        c.last_iv = (int)pt.bounds.size() - 1;
+        hipLaunchKernelGGL(mp::path_compose_kernel, dim3((unsigned)n_waves), dim3(mp::kWave), 0,
+                           sg.s, c);
+        MP_HIP(hipGetLastError());
     }
 
     int64_t forward_advance(int64_t n) override
@@ -265,22 +390,29 @@ struct AdjointRun final : AdjointBase {
         require(begun, "mp_adjoint_begin must come first");
         const int64_t target = std::min(N, fwd_done + std::max<int64_t>(n, 0));
         MP_HIP(hipSetDevice(device));
-        fwd_ms += timed([&] {
-            mp::PathArgs<T> a = args();
-            while (fwd_done < target) {
-                // launches start on checkpoints (multiples of C, hence of 4)
-                const int64_t n0 = fwd_done, n1 = std::min(N, n0 + spl);
+        mp::PathArgs<T> a = args();
+        while (fwd_done < target) {
+            // launches start on checkpoints (multiples of C, hence of 4)
+            const int64_t n0 = fwd_done, n1 = std::min(N, n0 + spl);
+            const int64_t j0 = n0 / C, j1 = (n1 + C - 1) / C;
+            fwd_ms += timed([&] {
                 table(n0, n1 - n0 + 2);
                 a.tab0 = n0;
                 a.n0 = n0;
                 a.n1 = n1;
+                a.ck_base = j0;
                 a.s_next = std::lower_bound(sched.begin(), sched.end(), n0) - sched.begin();
                 hipLaunchKernelGGL(mp::path_forward_kernel<T>, dim3((unsigned)n_waves),
                                    dim3(mp::kWave), 0, sg.s, a);
                 MP_HIP(hipGetLastError());
-                fwd_done = n1;
-            }
-        });
+            });
+            if (recording)
+                rev_ms += timed([&] {
+                    for (int64_t b = j0; b < j1; b += batch_segs)
+                        record_batch(b, std::min(j1, b + batch_segs), j0);
+                });
+            fwd_done = n1;
+        }
         if (fwd_done == N) fwd_finished = true;
         return fwd_done;
     }
@@ -309,58 +441,55 @@ struct AdjointRun final : AdjointBase {
     void reverse_begin(const double *w, double wf) override
     {
         require(begun, "mp_adjoint_begin must come first");
+        require(recording, "the forward pass did not record the gradient (record_gradient = 0)");
         if (!fwd_finished) forward_advance(N);
         require(S == 0 || w != nullptr, "grad_ens_mz is required when there are samples");
-        MP_HIP(hipSetDevice(device));
-        // J = sum_s w_s Ms/P sum_p m_z,p(k_s) + wf/P sum_p m_z,p(N)
-        std::vector<double> c((size_t)std::max<int64_t>(S, 1), 0.0);
-        double lz0 = wf / (double)P;
+        // J = sum_s w_s Ms/P sum_p m_z,p(k_s) + wf/P sum_p m_z,p(N): the
+        // weight of step k goes into lambda_z where the scan reaches step k
+        std::vector<double> in(pt.bounds.size(), 0.0);
+        lambda_z0 = wf / (double)P;
         for (int64_t s = 0; s < S; ++s) {
-            c[s] = w[s] * sp.magnetisation / (double)P;
-            if (sched[s] == N) lz0 += c[s];
+            const double c = w[s] * sp.magnetisation / (double)P;
+            if (sched[s] == N) {
+                lambda_z0 += c;
+            } else {
+                const size_t k = (size_t)(std::lower_bound(pt.bounds.begin(), pt.bounds.end(),
+                                                           sched[s]) - pt.bounds.begin());
+                in[k] += c;   // sched[s] is a boundary (or 0)
+            }
         }
-        MP_HIP(hipMemcpy(c_sample.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
-        std::vector<T> l(3 * P, T(0));
-        for (int64_t p = 0; p < P; ++p) l[2 * P + p] = (T)lz0;
-        MP_HIP(hipMemcpy(lam.p, l.data(), l.size() * sizeof(T), hipMemcpyHostToDevice));
-        MP_HIP(hipMemset(g.p, 0, g.n * sizeof(double)));
-        seg_next = n_ck - 1;
+        MP_HIP(hipSetDevice(device));
+        MP_HIP(hipMemcpy(inj.p, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice));
         rev_begun = true;
+        rev_done = false;
     }
 
-    int64_t steps_left() const { return seg_next < 0 ? 0 : std::min(N, (seg_next + 1) * C); }
-
-    int64_t reverse_advance(int64_t n) override
+    int64_t reverse_advance(int64_t) override
     {
         require(rev_begun, "mp_adjoint_reverse_begin must come first");
-        const int64_t stop = std::max<int64_t>(0, steps_left() - std::max<int64_t>(n, 0));
-        const int64_t segs = std::max<int64_t>(1, spl / C);
-        MP_HIP(hipSetDevice(device));
-        rev_ms += timed([&] {
-            mp::PathArgs<T> a = args();
-            while (seg_next >= 0 && steps_left() > stop) {
-                const int64_t hi = seg_next, lo = std::max<int64_t>(0, hi - segs + 1);
-                const int64_t n_lo = lo * C, n_end = std::min(N, (hi + 1) * C);
-                table(n_lo, n_end - n_lo + 2);
-                a.tab0 = n_lo;
-                a.seg_lo = lo;
-                a.seg_hi = hi;
-                // last sample strictly before n_end (samples at n_end are in lambda)
-                a.s_last = (int64_t)(std::lower_bound(sched.begin(), sched.end(), n_end) -
-                                     sched.begin()) - 1;
-                hipLaunchKernelGGL(mp::path_reverse_kernel<T>, dim3((unsigned)n_waves),
-                                   dim3(mp::kWave), 0, sg.s, a);
+        if (!rev_done) {
+            MP_HIP(hipSetDevice(device));
+            rev_ms += timed([&] {
+                mp::ScanArgs a{};
+                a.n = (int)P;
+                a.n_intervals = (int)pt.bounds.size();
+                a.intervals = intervals.p;
+                a.inj = inj.p;
+                a.lambda_z0 = lambda_z0;
+                a.g = g.p;
+                hipLaunchKernelGGL(mp::path_scan_kernel, dim3((unsigned)n_waves), dim3(mp::kWave),
+                                   0, sg.s, a);
                 MP_HIP(hipGetLastError());
-                seg_next = lo - 1;
-            }
-        });
-        return steps_left();
+            });
+            rev_done = true;
+        }
+        return 0;
     }
 
     void reverse_result(double grad[4]) override
     {
         require(rev_begun, "mp_adjoint_reverse_begin must come first");
-        if (seg_next >= 0) reverse_advance(N);
+        reverse_advance(0);
         std::vector<double> hg(g.n);
         MP_HIP(hipMemcpy(hg.data(), g.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int i = 0; i < 4; ++i) grad[i] = 0.0;
@@ -403,11 +532,11 @@ int mp_adjoint_create(const mp_adjoint_spec *spec, mp_adjoint **out)
 
 void mp_adjoint_destroy(mp_adjoint *a) { delete a; }
 
-int mp_adjoint_begin(mp_adjoint *a, const double theta[4])
+int mp_adjoint_begin(mp_adjoint *a, const double theta[4], int record_gradient)
 {
     return guarded([&] {
         require(a && theta, "handle and theta are required");
-        a->impl->begin(theta);
+        a->impl->begin(theta, record_gradient != 0);
     });
 }
 

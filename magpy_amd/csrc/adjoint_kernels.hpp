@@ -10,14 +10,17 @@
 //   s_k = sin(2 pi frac(k f dt)),  f dt the field cycles per step in SI units,
 // so the phase does not move with the parameters.
 //
-// The forward pass stores the state every C steps (checkpoints, [ck][3][P]).
-// The reverse pass walks the segments backwards: it recomputes a segment's
-// states from its checkpoint into a per-lane scratch ([C][3][P], each lane
-// reads back only what it wrote), then carries the adjoint lambda = dJ/dm
-// back through the segment and accumulates dJ/d(dt, sig, alpha, happ) in
-// FP64.  The objective is linear in the sampled m_z: J = sum_s c_s
-// sum_p m_z,p(k_s) + c_f sum_p m_z,p(N), so lambda_z picks up c_s at each
-// sample step and c_f at the end.
+// The forward pass stores the state every C steps (checkpoints).  The
+// objective is linear in the sampled m_z: J = sum_s c_s sum_p m_z,p(k_s) +
+// c_f sum_p m_z,p(N), so the adjoint lambda = dJ/dm picks up c_s at each
+// sample step and c_f at the end, and between sample steps it is a linear
+// map of its end value.  Those maps (transfers, below) are computed right
+// after each forward batch, for all segments of the batch in parallel: a
+// lane per (segment, particle) recomputes its segment from the checkpoint
+// into scratch ([C][3][lanes], read back only by the lane that wrote it) and
+// carries three basis adjoints back through it.  They are folded into one
+// transfer per sampling interval, and a reverse pass with given weights is
+// then a short scan over the intervals.
 //
 // Transposed derivatives of L used by the reverse step (x a 3-vector):
 //   (dL/dv)^T x     = m x x + alpha (|m|^2 x - m (m.x))
@@ -61,25 +64,28 @@ struct PathArgs {
     int64_t tab0;
     int64_t n_steps;                // N
     int64_t ck_every;               // C (a multiple of 4)
-    T *ck;                          // [N / C rounded up][3][n]
-    T *scratch;                     // [C][3][n]
+    T *ck;                          // checkpoints [segment - ck_base][3][n]
+    int64_t ck_base;
     uint32_t *status;               // bit 0: non-finite state
-    // forward: steps [n0, n1); reverse: segments [seg_lo, seg_hi] downwards
-    int64_t n0, n1;
-    int64_t seg_lo, seg_hi;
+    int64_t n0, n1;                 // steps [n0, n1) of this launch
     // sampling (ascending steps in [0, N])
     const int64_t *sched;
     int64_t S;
-    int64_t s_next;                 // forward: first sample with sched >= n0
-    double *mz_partial;             // forward: [S + 1][n_waves] wave sums (row S: final)
+    int64_t s_next;                 // first sample with sched >= n0
+    double *mz_partial;             // [S + 1][n_waves] wave sums (row S: final)
     int n_waves;
-    // reverse
-    const double *c_sample;         // [S] weight of sum_p m_z,p(k_s)
-    double c_final;
-    T *lx, *ly, *lz;                // lambda carried between launches
-    double *g;                      // [4][n]: dJ/d(dt, sig, alpha, happ) per particle
-    int64_t s_last;                 // reverse: last sample with sched <= this launch's end
 };
+
+// w of step n from the unscaled normals of its 4-step group (no dynamic
+// register indexing)
+template <typename T>
+__device__ inline V3<T> group_w(const T z[12], int i)
+{
+    if (i == 0) return V3<T>{z[0], z[1], z[2]};
+    if (i == 1) return V3<T>{z[3], z[4], z[5]};
+    if (i == 2) return V3<T>{z[6], z[7], z[8]};
+    return V3<T>{z[9], z[10], z[11]};
+}
 
 // field h(m, s) and v = dt h + sig w
 template <typename T>
@@ -165,7 +171,7 @@ __global__ __launch_bounds__(kWave) void path_forward_kernel(PathArgs<T> a)
     T z[12];
     for (int64_t n = a.n0; n < a.n1; ++n) {
         if (n == ck_n) {
-            if (live) store3(a.ck + (size_t)ck_i * 3 * P, P, p, m);
+            if (live) store3(a.ck + (size_t)(ck_i - a.ck_base) * 3 * P, P, p, m);
             ++ck_i;
             ck_n += a.ck_every;
         }
@@ -191,121 +197,284 @@ __global__ __launch_bounds__(kWave) void path_forward_kernel(PathArgs<T> a)
     }
 }
 
-// Reverse sweep over segments seg_hi, seg_hi - 1, ..., seg_lo.  On entry
-// lambda (lx, ly, lz) is dJ/dm at the end of segment seg_hi, including the
-// weights of samples at that step.
+// ------------------------------------------------------------------ reverse
+// The adjoint is linear in lambda: over any stretch [x, y) of steps
+//   lambda_x = Phi^T lambda_y,   g += G^T lambda_y
+// with Phi^T (3 x 3) and G^T (4 x 3) independent of the objective's weights.
+// The transfer kernel computes them for every stretch between consecutive
+// sample steps inside each segment, all segments of a batch in parallel (one
+// lane per (segment, particle)); the compose kernel folds the stretches of one
+// sampling interval into one transfer; the scan kernel runs the weighted
+// reverse pass over the intervals.  Transfers compose as
+//   (Phi_A^T, G_A^T) then (Phi_B^T, G_B^T)  ->  (Phi_A^T Phi_B^T, G_A^T Phi_B^T + G_B^T)
+// for A = [x, y) followed by B = [y, z).  A transfer is 21 doubles:
+// Phi^T row-major (9), then G^T row-major (12).
+constexpr int kTransfer = 21;
+
+// Forward intermediates of one step needed by its adjoint
 template <typename T>
-__global__ __launch_bounds__(kWave) void path_reverse_kernel(PathArgs<T> a)
+struct StepFwd {
+    V3<T> m, v1, mt, v2, w;
+    T s1, s2;
+};
+
+template <typename T>
+__device__ inline StepFwd<T> step_forward(const V3<T> &m, const V3<T> &e, const V3<T> &w, T dt,
+                                          T sig, T al, T ha, T s1, T s2)
 {
-    const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = p0 < a.n;
-    const int p = live ? p0 : a.n - 1;
-    const size_t P = (size_t)a.n;
+    StepFwd<T> f;
+    f.m = m;
+    f.w = w;
+    f.s1 = s1;
+    f.s2 = s2;
+    f.v1 = path_v(m, e, w, dt, sig, ha, s1);
+    const V3<T> a1 = path_L(m, f.v1, al);
+    f.mt = V3<T>{m.x + a1.x, m.y + a1.y, m.z + a1.z};
+    f.v2 = path_v(f.mt, e, w, dt, sig, ha, s2);
+    return f;
+}
+
+// lambda_n = (dm_{n+1}/dm_n)^T lambda_{n+1}, and g += (dm_{n+1}/d(dt, sig,
+// alpha, happ))^T lambda_{n+1}, for m_{n+1} = m + (L(m, v1) + L(mt, v2)) / 2.
+template <typename T>
+__device__ inline V3<T> step_adjoint(const StepFwd<T> &f, const V3<T> &e, T dt, T al, T ha,
+                                     const V3<T> &lam, double g[4])
+{
+    const V3<T> &mm = f.m, &v1 = f.v1, &mt = f.mt, &v2 = f.v2, &w = f.w;
+    const V3<T> ab{T(0.5) * lam.x, T(0.5) * lam.y, T(0.5) * lam.z};
+    // a2 = L(mt, v2)
+    const T tt = dot(mt, mt), tv2 = dot(mt, v2);
+    const T v2b = dot(v2, ab), tb = dot(mt, ab);
+    const V3<T> vx2 = cross(v2, ab), tx2 = cross(mt, ab);
+    V3<T> mtb{-vx2.x + al * (T(2) * mt.x * v2b - tv2 * ab.x - v2.x * tb),
+              -vx2.y + al * (T(2) * mt.y * v2b - tv2 * ab.y - v2.y * tb),
+              -vx2.z + al * (T(2) * mt.z * v2b - tv2 * ab.z - v2.z * tb)};
+    const V3<T> v2bar{tx2.x + al * (tt * ab.x - mt.x * tb), tx2.y + al * (tt * ab.y - mt.y * tb),
+                      tx2.z + al * (tt * ab.z - mt.z * tb)};
+    g[2] += (double)(v2b * tt - tb * tv2);
+    // v2 = dt h(mt, s2) + sig w
+    const T ev2 = dot(e, v2bar);
+    const T mte = dot(mt, e);
+    mtb.x += dt * e.x * ev2;
+    mtb.y += dt * e.y * ev2;
+    mtb.z += dt * e.z * ev2;
+    g[0] += (double)(mte * ev2 + ha * f.s2 * v2bar.z);
+    g[1] += (double)dot(w, v2bar);
+    g[3] += (double)(dt * f.s2 * v2bar.z);
+    // mt = m + a1
+    const V3<T> a1b{ab.x + mtb.x, ab.y + mtb.y, ab.z + mtb.z};
+    V3<T> mb{lam.x + mtb.x, lam.y + mtb.y, lam.z + mtb.z};
+    // a1 = L(m, v1)
+    const T mm2 = dot(mm, mm), mv1 = dot(mm, v1);
+    const T v1b = dot(v1, a1b), mb1 = dot(mm, a1b);
+    const V3<T> vx1 = cross(v1, a1b), mx1 = cross(mm, a1b);
+    mb.x += -vx1.x + al * (T(2) * mm.x * v1b - mv1 * a1b.x - v1.x * mb1);
+    mb.y += -vx1.y + al * (T(2) * mm.y * v1b - mv1 * a1b.y - v1.y * mb1);
+    mb.z += -vx1.z + al * (T(2) * mm.z * v1b - mv1 * a1b.z - v1.z * mb1);
+    const V3<T> v1bar{mx1.x + al * (mm2 * a1b.x - mm.x * mb1), mx1.y + al * (mm2 * a1b.y - mm.y * mb1),
+                      mx1.z + al * (mm2 * a1b.z - mm.z * mb1)};
+    g[2] += (double)(v1b * mm2 - mb1 * mv1);
+    // v1 = dt h(m, s1) + sig w
+    const T ev1 = dot(e, v1bar);
+    const T me = dot(mm, e);
+    mb.x += dt * e.x * ev1;
+    mb.y += dt * e.y * ev1;
+    mb.z += dt * e.z * ev1;
+    g[0] += (double)(me * ev1 + ha * f.s1 * v1bar.z);
+    g[1] += (double)dot(w, v1bar);
+    g[3] += (double)(dt * f.s1 * v1bar.z);
+    return mb;
+}
+
+template <typename T>
+struct TransferArgs {
+    int n;                          // particles
+    const T *ex, *ey, *ez, *sig;
+    const uint32_t *key0, *key1;
+    uint32_t tag;
+    T dt, alpha, happ;
+    const T *ftab;                  // ftab[i] = s_{tab0 + i}
+    int64_t tab0;
+    int64_t n_steps, ck_every;
+    const T *ck;                    // checkpoints of this batch's segments [seg][3][n]
+    int64_t seg0, n_segs;           // global index of the first segment, segments in batch
+    T *scratch;                     // [C][3][n_segs * n]
+    const int32_t *seg_first_piece, *seg_n_pieces;   // per segment of the batch
+    const int64_t *piece_start;     // first step of each piece (run-wide index)
+    int64_t piece_base;             // run-wide index of the batch's first piece
+    double *pieces;                 // [piece - piece_base][21][n]
+};
+
+// One lane per (segment, particle), particle fastest.
+template <typename T>
+__global__ __launch_bounds__(kWave) void path_transfer_kernel(TransferArgs<T> a)
+{
+    const int64_t lanes = a.n_segs * (int64_t)a.n;
+    const int64_t gid0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = gid0 < lanes;
+    const int64_t gid = live ? gid0 : lanes - 1;
+    const int sl = (int)(gid / a.n), p = (int)(gid % a.n);
+    const size_t P = (size_t)a.n, L = (size_t)lanes;
+    const int64_t seg = a.seg0 + sl;
+    const int64_t n_a = seg * a.ck_every;
+    const int64_t n_b = n_a + a.ck_every < a.n_steps ? n_a + a.ck_every : a.n_steps;
     const V3<T> e{a.ex[p], a.ey[p], a.ez[p]};
     const T sig = a.sig[p], dt = a.dt, al = a.alpha, ha = a.happ;
     PhiloxKeys keys;
     keys.init(a.key0[p], a.key1[p]);
-    V3<T> lam{a.lx[p], a.ly[p], a.lz[p]};
-    double g_dt = 0.0, g_sig = 0.0, g_al = 0.0, g_ha = 0.0;
-    int64_t s = a.s_last;   // samples [0, s] have sched <= current step
     T z[12];
-    for (int64_t seg = a.seg_hi; seg >= a.seg_lo; --seg) {
-        const int64_t n_a = seg * a.ck_every;
-        const int64_t n_b = n_a + a.ck_every < a.n_steps ? n_a + a.ck_every : a.n_steps;
-        // 1. recompute the segment's states m_{n_a} .. m_{n_b - 1}
-        V3<T> m = load3(a.ck + (size_t)seg * 3 * P, P, p);
-        for (int64_t n = n_a; n < n_b; ++n) {
-            const int i = (int)(n & 3);
-            if (i == 0) path_group_normals<T>(n >> 2, a.tag, keys, z);
-            T w0 = z[0], w1 = z[1], w2 = z[2];
-            if (i == 1) { w0 = z[3]; w1 = z[4]; w2 = z[5]; }
-            else if (i == 2) { w0 = z[6]; w1 = z[7]; w2 = z[8]; }
-            else if (i == 3) { w0 = z[9]; w1 = z[10]; w2 = z[11]; }
-            if (live) store3(a.scratch + (size_t)(n - n_a) * 3 * P, P, p, m);
-            const T s1 = a.ftab[n + 1 - a.tab0], s2 = a.ftab[n + 2 - a.tab0];
-            m = path_step(m, e, V3<T>{w0, w1, w2}, dt, sig, al, ha, s1, s2);
+    // 1. the segment's states m_{n_a} .. m_{n_b - 1}
+    V3<T> m = load3(a.ck + (size_t)sl * 3 * P, P, p);
+    for (int64_t n = n_a; n < n_b; ++n) {
+        const int i = (int)(n & 3);
+        if (i == 0) path_group_normals<T>(n >> 2, a.tag, keys, z);
+        const V3<T> w = group_w(z, i);
+        if (live) store3(a.scratch + (size_t)(n - n_a) * 3 * L, L, (int)gid, m);
+        m = path_step(m, e, w, dt, sig, al, ha, a.ftab[n + 1 - a.tab0], a.ftab[n + 2 - a.tab0]);
+    }
+    // 2. three adjoint channels (lambda = e_x, e_y, e_z at the end of each
+    //    piece) back through the segment; a piece ends at every sample step
+    V3<T> lam[3];
+    double g[3][4];
+    auto reset = [&]() {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lam[c] = V3<T>{T(c == 0), T(c == 1), T(c == 2)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[c][q] = 0.0;
         }
-        // 2. carry lambda back through steps n_b - 1 .. n_a
-        int64_t have = -1;
-        for (int64_t n = n_b - 1; n >= n_a; --n) {
-            if ((n >> 2) != have) {
-                have = n >> 2;
-                path_group_normals<T>(have, a.tag, keys, z);
+    };
+    reset();
+    int piece = a.seg_first_piece[sl] + a.seg_n_pieces[sl] - 1;
+    int64_t piece_begin = a.piece_start[piece];
+    int64_t have = -1;
+    for (int64_t n = n_b - 1; n >= n_a; --n) {
+        if ((n >> 2) != have) {
+            have = n >> 2;
+            path_group_normals<T>(have, a.tag, keys, z);
+        }
+        const V3<T> w = group_w(z, (int)(n & 3));
+        const V3<T> mm = load3(a.scratch + (size_t)(n - n_a) * 3 * L, L, (int)gid);
+        const StepFwd<T> f = step_forward(mm, e, w, dt, sig, al, ha, a.ftab[n + 1 - a.tab0],
+                                          a.ftab[n + 2 - a.tab0]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lam[c] = step_adjoint(f, e, dt, al, ha, lam[c], g[c]);
+        if (n == piece_begin) {
+            if (live) {
+                double *out = a.pieces + (size_t)(piece - a.piece_base) * kTransfer * P + p;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    out[(3 * r + 0) * P] = (double)(r == 0 ? lam[0].x : r == 1 ? lam[0].y : lam[0].z);
+                    out[(3 * r + 1) * P] = (double)(r == 0 ? lam[1].x : r == 1 ? lam[1].y : lam[1].z);
+                    out[(3 * r + 2) * P] = (double)(r == 0 ? lam[2].x : r == 1 ? lam[2].y : lam[2].z);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) out[(9 + 3 * q + c) * P] = g[c][q];
             }
-            const int i = (int)(n & 3);
-            T w0 = z[0], w1 = z[1], w2 = z[2];
-            if (i == 1) { w0 = z[3]; w1 = z[4]; w2 = z[5]; }
-            else if (i == 2) { w0 = z[6]; w1 = z[7]; w2 = z[8]; }
-            else if (i == 3) { w0 = z[9]; w1 = z[10]; w2 = z[11]; }
-            const V3<T> w{w0, w1, w2};
-            // padding lanes (p0 >= n) read particle n - 1's slot; their results are dropped
-            const V3<T> mm = load3(a.scratch + (size_t)(n - n_a) * 3 * P, P, p);
-            const T s1 = a.ftab[n + 1 - a.tab0], s2 = a.ftab[n + 2 - a.tab0];
-            // forward intermediates
-            const V3<T> v1 = path_v(mm, e, w, dt, sig, ha, s1);
-            const V3<T> a1 = path_L(mm, v1, al);
-            const V3<T> mt{mm.x + a1.x, mm.y + a1.y, mm.z + a1.z};
-            const V3<T> v2 = path_v(mt, e, w, dt, sig, ha, s2);
-            // m' = m + (a1 + a2) / 2
-            const V3<T> ab{T(0.5) * lam.x, T(0.5) * lam.y, T(0.5) * lam.z};
-            // a2 = L(mt, v2)
-            const T tt = dot(mt, mt), tv2 = dot(mt, v2);
-            const T v2b = dot(v2, ab), tb = dot(mt, ab);
-            const V3<T> vx2 = cross(v2, ab), tx2 = cross(mt, ab);
-            V3<T> mtb{-vx2.x + al * (T(2) * mt.x * v2b - tv2 * ab.x - v2.x * tb),
-                      -vx2.y + al * (T(2) * mt.y * v2b - tv2 * ab.y - v2.y * tb),
-                      -vx2.z + al * (T(2) * mt.z * v2b - tv2 * ab.z - v2.z * tb)};
-            const V3<T> v2bar{tx2.x + al * (tt * ab.x - mt.x * tb),
-                              tx2.y + al * (tt * ab.y - mt.y * tb),
-                              tx2.z + al * (tt * ab.z - mt.z * tb)};
-            g_al += (double)(dot(v2, ab) * tt - tb * tv2);
-            // v2 = dt h(mt, s2) + sig w
-            const T ev2 = dot(e, v2bar);
-            const T mte = dot(mt, e);
-            mtb.x += dt * e.x * ev2;
-            mtb.y += dt * e.y * ev2;
-            mtb.z += dt * e.z * ev2;
-            g_dt += (double)(mte * ev2 + ha * s2 * v2bar.z);
-            g_sig += (double)dot(w, v2bar);
-            g_ha += (double)(dt * s2 * v2bar.z);
-            // mt = m + a1
-            const V3<T> a1b{ab.x + mtb.x, ab.y + mtb.y, ab.z + mtb.z};
-            V3<T> mb{lam.x + mtb.x, lam.y + mtb.y, lam.z + mtb.z};
-            // a1 = L(m, v1)
-            const T mm2 = dot(mm, mm), mv1 = dot(mm, v1);
-            const T v1b = dot(v1, a1b), mb1 = dot(mm, a1b);
-            const V3<T> vx1 = cross(v1, a1b), mx1 = cross(mm, a1b);
-            mb.x += -vx1.x + al * (T(2) * mm.x * v1b - mv1 * a1b.x - v1.x * mb1);
-            mb.y += -vx1.y + al * (T(2) * mm.y * v1b - mv1 * a1b.y - v1.y * mb1);
-            mb.z += -vx1.z + al * (T(2) * mm.z * v1b - mv1 * a1b.z - v1.z * mb1);
-            const V3<T> v1bar{mx1.x + al * (mm2 * a1b.x - mm.x * mb1),
-                              mx1.y + al * (mm2 * a1b.y - mm.y * mb1),
-                              mx1.z + al * (mm2 * a1b.z - mm.z * mb1)};
-            g_al += (double)(v1b * mm2 - mb1 * mv1);
-            // v1 = dt h(m, s1) + sig w
-            const T ev1 = dot(e, v1bar);
-            const T me = dot(mm, e);
-            mb.x += dt * e.x * ev1;
-            mb.y += dt * e.y * ev1;
-            mb.z += dt * e.z * ev1;
-            g_dt += (double)(me * ev1 + ha * s1 * v1bar.z);
-            g_sig += (double)dot(w, v1bar);
-            g_ha += (double)(dt * s1 * v1bar.z);
-            lam = mb;
-            // weights of samples taken at step n
-            while (s >= 0 && a.sched[s] == n) lam.z += (T)a.c_sample[s--];
+            reset();
+            if (piece > a.seg_first_piece[sl]) piece_begin = a.piece_start[--piece];
         }
     }
-    if (live) {
-        a.lx[p] = lam.x;
-        a.ly[p] = lam.y;
-        a.lz[p] = lam.z;
-        a.g[p] += g_dt;
-        a.g[P + p] += g_sig;
-        a.g[2 * P + p] += g_al;
-        a.g[3 * P + p] += g_ha;
-        if (!(isfinite(lam.x) && isfinite(lam.y) && isfinite(lam.z))) a.status[p] |= 2u;
+}
+
+// Fold a batch's pieces (forward order) into the running transfer of the
+// open sampling interval; a piece that starts an interval first closes the
+// previous one into intervals[iv - 1].  acc [21][n] carries the open
+// interval between batches.  close_last writes the open interval to
+// intervals[last_iv] at the end (the last batch of the run).
+struct ComposeArgs {
+    int n;
+    int n_pieces;
+    const double *pieces;           // [piece][21][n]
+    const int32_t *piece_new;       // 1: the piece starts interval piece_iv
+    const int32_t *piece_iv;
+    double *acc;                    // [21][n]
+    double *intervals;              // [interval][21][n]
+    int close_last;
+    int last_iv;
+};
+
+__global__ __launch_bounds__(kWave) void path_compose_kernel(ComposeArgs a)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const size_t P = (size_t)a.n;
+    double acc[kTransfer];
+#pragma unroll
+    for (int q = 0; q < kTransfer; ++q) acc[q] = a.acc[q * P + p];
+    for (int k = 0; k < a.n_pieces; ++k) {
+        double x[kTransfer];
+#pragma unroll
+        for (int q = 0; q < kTransfer; ++q) x[q] = a.pieces[((size_t)k * kTransfer + q) * P + p];
+        if (a.piece_new[k]) {
+            if (a.piece_iv[k] > 0)
+#pragma unroll
+                for (int q = 0; q < kTransfer; ++q)
+                    a.intervals[((size_t)(a.piece_iv[k] - 1) * kTransfer + q) * P + p] = acc[q];
+#pragma unroll
+            for (int q = 0; q < kTransfer; ++q) acc[q] = x[q];
+        } else {
+            // acc then x:  Phi = Phi_acc Phi_x,  G = G_acc Phi_x + G_x  (transposes)
+            double r[kTransfer];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    r[3 * i + j] = acc[3 * i] * x[j] + acc[3 * i + 1] * x[3 + j] + acc[3 * i + 2] * x[6 + j];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    r[9 + 3 * q + j] = acc[9 + 3 * q] * x[j] + acc[9 + 3 * q + 1] * x[3 + j] +
+                                       acc[9 + 3 * q + 2] * x[6 + j] + x[9 + 3 * q + j];
+#pragma unroll
+            for (int q = 0; q < kTransfer; ++q) acc[q] = r[q];
+        }
     }
+    if (a.close_last) {
+#pragma unroll
+        for (int q = 0; q < kTransfer; ++q)
+            a.intervals[((size_t)a.last_iv * kTransfer + q) * P + p] = acc[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kTransfer; ++q) a.acc[q * P + p] = acc[q];
+}
+
+// Weighted reverse pass over the sampling intervals, last to first:
+// g += G_i^T lambda; lambda = Phi_i^T lambda + inj[i] e_z (inj[i]: weights of
+// the samples at the interval's first step).
+struct ScanArgs {
+    int n;
+    int n_intervals;
+    const double *intervals;        // [interval][21][n]
+    const double *inj;              // [interval]
+    double lambda_z0;               // weights at step N
+    double *g;                      // [4][n] out
+};
+
+__global__ __launch_bounds__(kWave) void path_scan_kernel(ScanArgs a)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const size_t P = (size_t)a.n;
+    double l0 = 0.0, l1 = 0.0, l2 = a.lambda_z0;
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = a.n_intervals - 1; i >= 0; --i) {
+        const double *t = a.intervals + (size_t)i * kTransfer * P + p;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            g[q] += t[(9 + 3 * q) * P] * l0 + t[(9 + 3 * q + 1) * P] * l1 + t[(9 + 3 * q + 2) * P] * l2;
+        const double n0 = t[0] * l0 + t[P] * l1 + t[2 * P] * l2;
+        const double n1 = t[3 * P] * l0 + t[4 * P] * l1 + t[5 * P] * l2;
+        const double n2 = t[6 * P] * l0 + t[7 * P] * l1 + t[8 * P] * l2 + a.inj[i];
+        l0 = n0;
+        l1 = n1;
+        l2 = n2;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a.g[q * P + p] = g[q];
 }
 
 // Field table: out[i] = sin(2 pi frac((base + i) fdt)), FP64 phase reduction.

@@ -559,8 +559,8 @@ void launch_product(int mode, const float *a32, const __bf16 *a_hi, const __bf16
         hipLaunchKernelGGL(mp::gemm_f32_kernel, grid, dim3(256), 0, s, a32, b32, c, ld, ld);
     } else {
         const int nrb = rows / mp::kBM, ncb = ld / mp::kBN;
-        hipLaunchKernelGGL(mp::gemm_bf16x3_v2_kernel, dim3((unsigned)(nrb * ncb)), dim3(256), 0,
-                           s, a_hi, a_lo, b_hi, b_lo, c, ld, ld, nrb, ncb);
+        hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel, dim3((unsigned)(nrb * ncb)), dim3(512),
+                           0, s, a_hi, a_lo, b_hi, b_lo, c, ld, ld, nrb, ncb);
     }
 }
 

@@ -62,33 +62,40 @@ __device__ inline void store_tile32(float *C, int ldc, int r0, int c0, const f32
 }
 
 // Ht = Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T with LDS staging.  Block
-// tile 128 rows x 192 columns (4 waves of 64 x 96 = 2 x 3 MFMA tiles), k in
-// stages of 32 double-buffered in LDS: while the waves run a stage's 36
-// MFMAs each, the block's next stage (A hi/lo 128 x 32, B hi/lo 192 x 32) is
-// already in flight from global memory, three stages ahead in registers.
-// Blocks are numbered so that the blocks one XCD runs (block ids b with
-// equal b % 8) share column panels of D, which then stay in that XCD's L2.
-// rows % 128 == 0, cols % 192 == 0, K % 192 == 0 (stages come in sixes).
-constexpr int kBM = 128, kBN = 192, kBK = 32, kLdsPad = 8;
-constexpr int kLdsRow = kBK + kLdsPad;  // bf16 per LDS row
+// tile 128 rows x 192 columns; each tile of 64 x 96 (2 x 3 MFMA tiles) is
+// computed by two waves (8 waves per block), one per 16-k half of every
+// 32-k stage, so each SIMD holds two waves and one can issue MFMAs while the
+// other waits on LDS or the barrier; the halves are added through LDS at the
+// end.  Stages are double-buffered in LDS and prefetched three stages ahead
+// in registers.  Blocks are numbered so that the blocks one XCD runs (block
+// ids b with equal b % 8) share column panels of D, which then stay in that
+// XCD's L2.  rows % 128 == 0, cols % 192 == 0, K % 192 == 0 (stages come in
+// sixes).  MFMAs are ordered so that six independent accumulators separate
+// dependent ones.
+constexpr int kBM = 128, kBN = 192, kBK = 32;
 
-__global__ __launch_bounds__(256) void gemm_bf16x3_v2_kernel(const __bf16 *__restrict__ a_hi,
-                                                             const __bf16 *__restrict__ a_lo,
-                                                             const __bf16 *__restrict__ b_hi,
-                                                             const __bf16 *__restrict__ b_lo,
-                                                             float *__restrict__ c, int K, int ldc,
-                                                             int n_row_blocks, int n_col_blocks)
+static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(const __bf16 *__restrict__ a_hi,
+                                                                 const __bf16 *__restrict__ a_lo,
+                                                                 const __bf16 *__restrict__ b_hi,
+                                                                 const __bf16 *__restrict__ b_lo,
+                                                                 float *__restrict__ c, int K,
+                                                                 int ldc, int n_row_blocks,
+                                                                 int n_col_blocks)
 {
-    __shared__ __attribute__((aligned(16))) __bf16 sa[2][2][kBM * kLdsRow];
-    __shared__ __attribute__((aligned(16))) __bf16 sb[2][2][kBN * kLdsRow];
+    constexpr int kLdsRow = kBK + 8;
+    constexpr int kPanelA = kBM * kLdsRow, kPanelB = kBN * kLdsRow;   // bf16 per panel
+    // one LDS array: [buffer][hi/lo] A panels, then B panels; reused at the
+    // end for the reduction of the two k halves
+    __shared__ __attribute__((aligned(16))) __bf16 lds[4 * kPanelA + 4 * kPanelB];
+    auto sa = [&](int buf, int h) { return lds + (2 * buf + h) * kPanelA; };
+    auto sb = [&](int buf, int h) { return lds + 4 * kPanelA + (2 * buf + h) * kPanelB; };
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
-    // XCD-aware block order: consecutive logical blocks share a column block
+    const int kh = wave >> 2, w4 = wave & 3, wr = w4 >> 1, wc = w4 & 1;
     const int total = n_row_blocks * n_col_blocks;
     int logical = blockIdx.x;
     if (total % 8 == 0) logical = (blockIdx.x % 8) * (total / 8) + blockIdx.x / 8;
     const int cb = logical / n_row_blocks, rb = logical % n_row_blocks;
     const int row0 = rb * kBM, col0 = cb * kBN;
-    const int wr = wave >> 1, wc = wave & 1;
     f32x16 acc[2][3];
 #pragma unroll
     for (int t = 0; t < 2; ++t)
@@ -97,26 +104,26 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_v2_kernel(const __bf16 *__res
 #pragma unroll
             for (int q = 0; q < 16; ++q) acc[t][u][q] = 0.f;
 
-    // global -> register staging, three stages ahead: A panel 128 rows x 4
-    // chunks of 8 bf16 (2 chunks per thread per part), B panel 192 rows x 4
-    // chunks (3 per thread per part).  Register set (s % 3) holds stage s.
+    // staging: A 512 chunks of 16 B per part (one per thread), B 768 (threads
+    // 0..255 take a second one); three stages in flight
     typedef __attribute__((ext_vector_type(4))) uint32_t u32x4v;
-    u32x4v ga[3][2][2], gb[3][2][3];
+    u32x4v ga[3][2], gb[3][2][2];
+    const bool two_b = tid < 256;
     const int n_stages = K / kBK;
     auto load_stage = [&](auto set_c, int st) {
         constexpr int R = decltype(set_c)::value;
         if (st >= n_stages) return;
         const int k0 = st * kBK;
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int ch = tid + 256 * i, row = ch >> 2, kc = ch & 3;
+        {
+            const int row = tid >> 2, kc = tid & 3;
             const size_t off = (size_t)(row0 + row) * K + k0 + 8 * kc;
-            ga[R][0][i] = *reinterpret_cast<const u32x4v *>(a_hi + off);
-            ga[R][1][i] = *reinterpret_cast<const u32x4v *>(a_lo + off);
+            ga[R][0] = *reinterpret_cast<const u32x4v *>(a_hi + off);
+            ga[R][1] = *reinterpret_cast<const u32x4v *>(a_lo + off);
         }
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int ch = tid + 256 * i, row = ch >> 2, kc = ch & 3;
+        for (int i = 0; i < 2; ++i) {
+            if (i == 1 && !two_b) break;
+            const int ch = tid + 512 * i, row = ch >> 2, kc = ch & 3;
             const size_t off = (size_t)(col0 + row) * K + k0 + 8 * kc;
             gb[R][0][i] = *reinterpret_cast<const u32x4v *>(b_hi + off);
             gb[R][1][i] = *reinterpret_cast<const u32x4v *>(b_lo + off);
@@ -124,54 +131,48 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_v2_kernel(const __bf16 *__res
     };
     auto store_stage = [&](auto set_c, auto buf_c) {
         constexpr int R = decltype(set_c)::value, B = decltype(buf_c)::value;
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int ch = tid + 256 * i, row = ch >> 2, kc = ch & 3;
+        {
+            const int row = tid >> 2, kc = tid & 3;
 #pragma unroll
             for (int h = 0; h < 2; ++h)
-                *reinterpret_cast<u32x4v *>(&sa[B][h][row * kLdsRow + 8 * kc]) = ga[R][h][i];
+                *reinterpret_cast<u32x4v *>(sa(B, h) + row * kLdsRow + 8 * kc) = ga[R][h];
         }
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int ch = tid + 256 * i, row = ch >> 2, kc = ch & 3;
+        for (int i = 0; i < 2; ++i) {
+            if (i == 1 && !two_b) break;
+            const int ch = tid + 512 * i, row = ch >> 2, kc = ch & 3;
 #pragma unroll
             for (int h = 0; h < 2; ++h)
-                *reinterpret_cast<u32x4v *>(&sb[B][h][row * kLdsRow + 8 * kc]) = gb[R][h][i];
+                *reinterpret_cast<u32x4v *>(sb(B, h) + row * kLdsRow + 8 * kc) = gb[R][h][i];
         }
     };
     auto compute = [&](auto buf_c) {
         constexpr int B = decltype(buf_c)::value;
+        const int kq = 16 * kh + 8 * (l >> 5);
+        bf16x8 xa[2][2], xb[2][3];
 #pragma unroll
-        for (int kk = 0; kk < kBK; kk += 16) {
-            const int kq = kk + 8 * (l >> 5);
-            bf16x8 xa[2][2], xb[2][3];
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                xa[h][t] = *reinterpret_cast<const bf16x8 *>(
+                    sa(B, h) + (64 * wr + 32 * t + (l & 31)) * kLdsRow + kq);
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                xb[h][u] = *reinterpret_cast<const bf16x8 *>(
+                    sb(B, h) + (96 * wc + 32 * u + (l & 31)) * kLdsRow + kq);
+#pragma unroll
+        for (int term = 0; term < 3; ++term)
 #pragma unroll
             for (int t = 0; t < 2; ++t)
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
-                    xa[h][t] = *reinterpret_cast<const bf16x8 *>(
-                        &sa[B][h][(64 * wr + 32 * t + (l & 31)) * kLdsRow + kq]);
-#pragma unroll
-            for (int u = 0; u < 3; ++u)
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-                    xb[h][u] = *reinterpret_cast<const bf16x8 *>(
-                        &sb[B][h][(96 * wc + 32 * u + (l & 31)) * kLdsRow + kq]);
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int u = 0; u < 3; ++u) {
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[0][t], xb[0][u], acc[t][u], 0, 0, 0);
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[0][t], xb[1][u], acc[t][u], 0, 0, 0);
-                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xa[1][t], xb[0][u], acc[t][u], 0, 0, 0);
-                }
-        }
+                for (int u = 0; u < 3; ++u)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        xa[term == 2 ? 1 : 0][t], xb[term == 1 ? 1 : 0][u], acc[t][u], 0, 0, 0);
     };
-    // stage s: compute from LDS buffer s % 2, then move stage s + 1 from
-    // register set (s + 1) % 3 into the other buffer and refill that set
-    // with stage s + 4.
     auto step = [&](auto s_c, int st) {
-        constexpr int S6 = decltype(s_c)::value;   // st % 6
+        constexpr int S6 = decltype(s_c)::value;
         compute(std::integral_constant<int, S6 % 2>{});
         if (st + 1 < n_stages)
             store_stage(std::integral_constant<int, (S6 + 1) % 3>{},
@@ -193,11 +194,30 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_v2_kernel(const __bf16 *__res
         step(std::integral_constant<int, 4>{}, st + 4);
         step(std::integral_constant<int, 5>{}, st + 5);
     }
+    // the loop ends on a barrier: the staging LDS is free for the reduction
+    // of the two k halves (4 tiles x 6144 floats = 96 KB)
+    float *red = reinterpret_cast<float *>(lds);
+    static_assert(sizeof(lds) >= 4 * 6144 * sizeof(float), "reduction buffer");
+    float *mine = red + (size_t)w4 * 6144;
+    if (kh == 1) {
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < 3; ++u)
-            store_tile32(c, ldc, row0 + 64 * wr + 32 * t, col0 + 96 * wc + 32 * u, acc[t][u]);
+            for (int u = 0; u < 3; ++u)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) mine[((t * 3 + u) * 16 + q) * 64 + l] = acc[t][u][q];
+    }
+    __syncthreads();
+    if (kh == 0) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[t][u][q] += mine[((t * 3 + u) * 16 + q) * 64 + l];
+                store_tile32(c, ldc, row0 + 64 * wr + 32 * t, col0 + 96 * wc + 32 * u, acc[t][u]);
+            }
+    }
 }
 
 // The same product in FP32 on MFMA (v_mfma_f32_32x32x2f32).  Each lane reads

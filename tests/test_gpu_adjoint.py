@@ -170,3 +170,15 @@ def test_backward_after_another_forward_raises():
         obj(torch.tensor(THETA * 1.01))
     with pytest.raises(RuntimeError):
         res["sar"].backward()
+
+
+def test_value_only_forward_has_no_reverse():
+    print("CASE adjoint_value_only", flush=True)
+    shape, ax, seeds = problem(64, 8)
+    r = runner(shape, ax, seeds, 1e10, 400, [0, 200, 400], 100, "fp32")
+    ens, fin, st = r.forward(THETA, record_gradient=False)
+    ens2, fin2, _ = r.forward(THETA, record_gradient=True)
+    assert np.array_equal(ens, ens2) and fin == fin2
+    r.forward(THETA, record_gradient=False)
+    with pytest.raises(ValueError):
+        r.reverse([0.0, 1.0, 1.0], 0.0)
