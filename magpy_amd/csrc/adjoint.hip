@@ -163,8 +163,14 @@ struct AdjointRun final : AdjointBase {
     int device;
     int64_t P, N, C, S, n_ck, spl, segs_per_block, batch_segs, max_batch_pieces;
     int n_waves;
-    StreamGuard sg;
-    DevBuf<T> ex, ey, ez, sig, mx, my, mz, ck, scratch, ftab;
+    // forward kernels on sg.s, the adjoint maps on tr.s: the two run
+    // concurrently (the forward pass holds one wave per 64 particles; the
+    // maps fill the rest of the machine)
+    StreamGuard sg, tr;
+    hipEvent_t ev_fwd[2] = {nullptr, nullptr}, ev_tr[2] = {nullptr, nullptr};
+    bool tr_pending[2] = {false, false};
+    int64_t ck_half = 0;   // checkpoint buffer of one block; two alternate
+    DevBuf<T> ex, ey, ez, sig, mx, my, mz, ck, scratch, ftab, ftab_tr;
     DevBuf<uint32_t> key0, key1, status;
     DevBuf<double> partial, pieces, acc, intervals, inj, g;
     DevBuf<int64_t> d_sched, d_piece_start;
@@ -232,17 +238,33 @@ struct AdjointRun final : AdjointBase {
         mz.alloc(P);
         status.alloc(P);
         g.alloc(4 * P);
-        ck.alloc((size_t)(segs_per_block + 1) * 3 * P);
+        ck_half = (segs_per_block + 1) * 3 * P;
+        ck.alloc((size_t)(2 * ck_half));
         ck_bytes = (int64_t)(ck.n * sizeof(T));
+        for (int b = 0; b < 2; ++b) {
+            MP_HIP(hipEventCreateWithFlags(&ev_fwd[b], hipEventDisableTiming));
+            MP_HIP(hipEventCreateWithFlags(&ev_tr[b], hipEventDisableTiming));
+        }
         partial.alloc((size_t)(S + 1) * n_waves);
         if (S) d_sched.upload(sched);
         ftab.alloc((size_t)spl + 2);
+        ftab_tr.alloc((size_t)spl + 2);
         d_seg_first.upload(pt.seg_first);
         d_seg_count.upload(pt.seg_count);
         d_piece_start.upload(pt.start);
         d_piece_new.upload(pt.is_new);
         d_piece_iv.upload(pt.iv);
         inj.alloc(pt.bounds.size());
+    }
+
+    ~AdjointRun() override
+    {
+        (void)hipStreamSynchronize(sg.s);
+        (void)hipStreamSynchronize(tr.s);
+        for (int b = 0; b < 2; ++b) {
+            if (ev_fwd[b]) (void)hipEventDestroy(ev_fwd[b]);
+            if (ev_tr[b]) (void)hipEventDestroy(ev_tr[b]);
+        }
     }
 
     // device buffers of the gradient recording, allocated on first use
@@ -286,25 +308,47 @@ struct AdjointRun final : AdjointBase {
         return a;
     }
 
-    void table(int64_t base, int64_t count)
+    void table(DevBuf<T> &tab, hipStream_t st, int64_t base, int64_t count)
     {
-        require((size_t)count <= ftab.n, "field table too small");
+        require((size_t)count <= tab.n, "field table too small");
         const double fdt = sp.field_frequency * sp.time_step;
         hipLaunchKernelGGL(mp::path_field_table_kernel<T>, dim3((unsigned)((count + 255) / 256)),
-                           dim3(256), 0, sg.s, fdt, base, count, ftab.p);
+                           dim3(256), 0, st, fdt, base, count, tab.p);
         MP_HIP(hipGetLastError());
     }
 
+    // timing events of launches in flight: (stream's accumulator, start, end)
+    struct Timing {
+        double *acc;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Timing> timings;
+
     template <typename F>
-    double timed(F &&f)
+    void timed(double &acc, hipStream_t st, F &&f)
     {
-        MP_HIP(hipEventRecord(sg.e0, sg.s));
+        Timing t{&acc, nullptr, nullptr};
+        MP_HIP(hipEventCreate(&t.e0));
+        MP_HIP(hipEventCreate(&t.e1));
+        timings.push_back(t);
+        MP_HIP(hipEventRecord(t.e0, st));
         f();
-        MP_HIP(hipEventRecord(sg.e1, sg.s));
+        MP_HIP(hipEventRecord(t.e1, st));
+    }
+
+    // wait for both streams and account the timed launches
+    void drain()
+    {
         MP_HIP(hipStreamSynchronize(sg.s));
-        float ms = 0.f;
-        MP_HIP(hipEventElapsedTime(&ms, sg.e0, sg.e1));
-        return ms;
+        MP_HIP(hipStreamSynchronize(tr.s));
+        for (Timing &t : timings) {
+            float ms = 0.f;
+            MP_HIP(hipEventElapsedTime(&ms, t.e0, t.e1));
+            *t.acc += ms;
+            (void)hipEventDestroy(t.e0);
+            (void)hipEventDestroy(t.e1);
+        }
+        timings.clear();
     }
 
     void begin(const double th[4], bool record) override
@@ -322,8 +366,11 @@ struct AdjointRun final : AdjointBase {
         MP_HIP(hipMemcpy(mx.p, hx.data(), P * sizeof(T), hipMemcpyHostToDevice));
         MP_HIP(hipMemcpy(my.p, hy.data(), P * sizeof(T), hipMemcpyHostToDevice));
         MP_HIP(hipMemcpy(mz.p, hz.data(), P * sizeof(T), hipMemcpyHostToDevice));
-        MP_HIP(hipMemset(status.p, 0, P * sizeof(uint32_t)));
-        MP_HIP(hipMemset(partial.p, 0, partial.n * sizeof(double)));
+        drain();
+        // cleared on the forward stream, ahead of its kernels
+        MP_HIP(hipMemsetAsync(status.p, 0, P * sizeof(uint32_t), sg.s));
+        MP_HIP(hipMemsetAsync(partial.p, 0, partial.n * sizeof(double), sg.s));
+        tr_pending[0] = tr_pending[1] = false;
         recording = record;
         if (recording) alloc_recording();
         begun = true;
@@ -335,10 +382,10 @@ struct AdjointRun final : AdjointBase {
 
     // transfers of segments [j0, j1) (all inside the current forward block,
     // whose first segment is ck_base), folded into the intervals
-    void record_batch(int64_t j0, int64_t j1, int64_t ck_base)
+    void record_batch(int64_t j0, int64_t j1, int64_t ck_base, const T *ck_block)
     {
         const int64_t n_a = j0 * C, n_end = std::min(N, j1 * C);
-        table(n_a, n_end - n_a + 2);
+        table(ftab_tr, tr.s, n_a, n_end - n_a + 2);
         const int64_t piece0 = pt.seg_first[(size_t)j0];
         const int64_t piece1 = j1 < n_ck ? pt.seg_first[(size_t)j1] : (int64_t)pt.start.size();
         mp::TransferArgs<T> t{};
@@ -353,11 +400,11 @@ struct AdjointRun final : AdjointBase {
         t.dt = (T)q[0];
         t.alpha = (T)q[2];
         t.happ = (T)q[3];
-        t.ftab = ftab.p;
+        t.ftab = ftab_tr.p;
         t.tab0 = n_a;
         t.n_steps = N;
         t.ck_every = C;
-        t.ck = ck.p + (size_t)(j0 - ck_base) * 3 * P;
+        t.ck = ck_block + (size_t)(j0 - ck_base) * 3 * P;
         t.seg0 = j0;
         t.n_segs = j1 - j0;
         t.scratch = scratch.p;
@@ -368,7 +415,7 @@ struct AdjointRun final : AdjointBase {
         t.pieces = pieces.p;
         const int64_t lanes = (j1 - j0) * P;
         hipLaunchKernelGGL(mp::path_transfer_kernel<T>, dim3((unsigned)((lanes + 63) / 64)),
-                           dim3(mp::kWave), 0, sg.s, t);
+                           dim3(mp::kWave), 0, tr.s, t);
         MP_HIP(hipGetLastError());
         mp::ComposeArgs c{};
         c.n = (int)P;
@@ -381,7 +428,7 @@ struct AdjointRun final : AdjointBase {
         c.close_last = j1 == n_ck ? 1 : 0;
         c.last_iv = (int)pt.bounds.size() - 1;
         hipLaunchKernelGGL(mp::path_compose_kernel, dim3((unsigned)n_waves), dim3(mp::kWave), 0,
-                           sg.s, c);
+                           tr.s, c);
         MP_HIP(hipGetLastError());
     }
 
@@ -395,24 +442,35 @@ struct AdjointRun final : AdjointBase {
             // launches start on checkpoints (multiples of C, hence of 4)
             const int64_t n0 = fwd_done, n1 = std::min(N, n0 + spl);
             const int64_t j0 = n0 / C, j1 = (n1 + C - 1) / C;
-            fwd_ms += timed([&] {
-                table(n0, n1 - n0 + 2);
+            const int buf = (int)((n0 / spl) & 1);
+            T *ck_block = ck.p + (size_t)buf * ck_half;
+            // the maps of block k - 2 must be done with this checkpoint buffer
+            if (tr_pending[buf]) MP_HIP(hipStreamWaitEvent(sg.s, ev_tr[buf], 0));
+            timed(fwd_ms, sg.s, [&] {
+                table(ftab, sg.s, n0, n1 - n0 + 2);
                 a.tab0 = n0;
                 a.n0 = n0;
                 a.n1 = n1;
+                a.ck = ck_block;
                 a.ck_base = j0;
                 a.s_next = std::lower_bound(sched.begin(), sched.end(), n0) - sched.begin();
                 hipLaunchKernelGGL(mp::path_forward_kernel<T>, dim3((unsigned)n_waves),
                                    dim3(mp::kWave), 0, sg.s, a);
                 MP_HIP(hipGetLastError());
             });
-            if (recording)
-                rev_ms += timed([&] {
+            MP_HIP(hipEventRecord(ev_fwd[buf], sg.s));
+            if (recording) {
+                MP_HIP(hipStreamWaitEvent(tr.s, ev_fwd[buf], 0));
+                timed(rev_ms, tr.s, [&] {
                     for (int64_t b = j0; b < j1; b += batch_segs)
-                        record_batch(b, std::min(j1, b + batch_segs), j0);
+                        record_batch(b, std::min(j1, b + batch_segs), j0, ck_block);
                 });
+                MP_HIP(hipEventRecord(ev_tr[buf], tr.s));
+                tr_pending[buf] = true;
+            }
             fwd_done = n1;
         }
+        drain();
         if (fwd_done == N) fwd_finished = true;
         return fwd_done;
     }
@@ -420,6 +478,7 @@ struct AdjointRun final : AdjointBase {
     void forward_result(double *ens, double *fin, uint32_t *st) override
     {
         if (!fwd_finished) forward_advance(N);
+        drain();
         std::vector<double> part(partial.n);
         MP_HIP(hipMemcpy(part.data(), partial.p, part.size() * sizeof(double),
                          hipMemcpyDeviceToHost));
@@ -459,6 +518,7 @@ struct AdjointRun final : AdjointBase {
             }
         }
         MP_HIP(hipSetDevice(device));
+        drain();
         MP_HIP(hipMemcpy(inj.p, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice));
         rev_begun = true;
         rev_done = false;
@@ -469,7 +529,7 @@ struct AdjointRun final : AdjointBase {
         require(rev_begun, "mp_adjoint_reverse_begin must come first");
         if (!rev_done) {
             MP_HIP(hipSetDevice(device));
-            rev_ms += timed([&] {
+            timed(rev_ms, tr.s, [&] {
                 mp::ScanArgs a{};
                 a.n = (int)P;
                 a.n_intervals = (int)pt.bounds.size();
@@ -478,9 +538,10 @@ struct AdjointRun final : AdjointBase {
                 a.lambda_z0 = lambda_z0;
                 a.g = g.p;
                 hipLaunchKernelGGL(mp::path_scan_kernel, dim3((unsigned)n_waves), dim3(mp::kWave),
-                                   0, sg.s, a);
+                                   0, tr.s, a);
                 MP_HIP(hipGetLastError());
             });
+            drain();
             rev_done = true;
         }
         return 0;
