@@ -366,6 +366,7 @@ def gradient_run(run, theta, sched_time, f, precision_tag, chunk=1 << 24, record
            "forward_s": time.perf_counter() - t0}
     if record:
         g_sar = run.reverse(w, 0.0)
+        out["particle_grad_sar"] = run.particle_gradients()
         g_sar[2] += dh
         out["grad_sar"] = g_sar
         out["grad_final_mz"] = run.reverse(np.zeros(len(w)), 1.0)
@@ -438,6 +439,21 @@ def grad_check(args):
                                         np.abs(fd["final_mz"])).tolist()
             line["rel_err_sar_fp64"] = (np.abs(r64["grad_sar"] - fd["sar"]) /
                                         np.abs(fd["sar"])).tolist()
+            # how concentrated the SAR gradient is: the 10 largest particle
+            # contributions' share of the total magnitude, and FP32 vs FP64
+            # with the top 1 % particles (by either precision) set aside
+            p32, p64 = r["particle_grad_sar"], r64["particle_grad_sar"]
+            share, rest = [], []
+            for i in range(4):
+                mag = np.abs(p64[:, i])
+                share.append(float(np.sort(mag)[-10:].sum() / mag.sum()))
+                k = max(1, P // 100)
+                out_ = np.union1d(np.argsort(mag)[-k:], np.argsort(np.abs(p32[:, i]))[-k:])
+                keep = np.setdiff1d(np.arange(P), out_)
+                a64, a32 = p64[keep, i].sum(), p32[keep, i].sum()
+                rest.append(float(abs(a32 - a64) / abs(a64)))
+            line["sar_grad_top10_share_fp64"] = share
+            line["sar_grad_fp32_vs_fp64_without_top1pct"] = rest
             line["rel_err_final_mz_fp64"] = (np.abs(r64["grad_final_mz"] - fd["final_mz"]) /
                                              np.abs(fd["final_mz"])).tolist()
         print(json.dumps(line), flush=True)

@@ -256,6 +256,9 @@ int mp_adjoint_reverse_begin(mp_adjoint *a, const double *grad_ens_mz, double gr
 int mp_adjoint_reverse_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_left);
 /* finish the reverse pass: grad_theta[4] = dJ/d(K, alpha, H, r) */
 int mp_adjoint_reverse_result(mp_adjoint *a, double grad_theta[4]);
+/* the same gradient split by particle: grad [P][4], the part of dJ/dtheta
+ * that flows through each particle's trajectory (they sum to grad_theta) */
+int mp_adjoint_particle_gradients(mp_adjoint *a, double *grad);
 /* device milliseconds of the forward kernels and of the gradient kernels
  * (transfers, folding, scan) so far, and the device bytes held for
  * checkpoints and segment scratch */

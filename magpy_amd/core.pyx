@@ -107,6 +107,7 @@ cdef extern from "magpy_amd.h" nogil:
     int mp_adjoint_reverse_begin(mp_adjoint *a, const double *grad_ens_mz, double grad_final_mz)
     int mp_adjoint_reverse_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_left)
     int mp_adjoint_reverse_result(mp_adjoint *a, double *grad_theta)
+    int mp_adjoint_particle_gradients(mp_adjoint *a, double *grad)
     int mp_adjoint_info(mp_adjoint *a, double *forward_ms, double *reverse_ms,
                         int64_t *checkpoint_bytes, int64_t *scratch_bytes)
     int mp_adjoint_reduced(const mp_adjoint_spec *spec, const double *theta, double *q,
@@ -700,6 +701,16 @@ cdef class Adjoint:
     def reverse(self, grad_ens_mz, double grad_final_mz):
         self.reverse_begin(grad_ens_mz, grad_final_mz)
         return self.reverse_result()
+
+    def particle_gradients(self):
+        """[P, 4]: each particle's part of the last reverse pass's gradient."""
+        out = np.zeros((self.n_particles, 4))
+        cdef double[:, ::1] ov = out
+        cdef int rc
+        with nogil:
+            rc = mp_adjoint_particle_gradients(self.a, &ov[0, 0])
+        _check(rc)
+        return out
 
     def info(self):
         cdef double fms = 0, rms = 0

@@ -108,6 +108,7 @@ struct AdjointBase {
     virtual void reverse_begin(const double *w, double wf) = 0;
     virtual int64_t reverse_advance(int64_t n) = 0;
     virtual void reverse_result(double grad[4]) = 0;
+    virtual void particle_gradients(double *out) = 0;
     double fwd_ms = 0, rev_ms = 0;
     int64_t ck_bytes = 0, scratch_bytes = 0;
 };
@@ -547,20 +548,30 @@ struct AdjointRun final : AdjointBase {
         return 0;
     }
 
-    void reverse_result(double grad[4]) override
+    // per particle: dJ/dtheta [P][4] through that particle's trajectory
+    void particle_gradients(double *out) override
     {
         require(rev_begun, "mp_adjoint_reverse_begin must come first");
         reverse_advance(0);
         std::vector<double> hg(g.n);
         MP_HIP(hipMemcpy(hg.data(), g.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (int i = 0; i < 4; ++i) grad[i] = 0.0;
         for (int64_t p = 0; p < P; ++p) {
             const double *d = dq.data() + p * kQ * 4;
+            for (int i = 0; i < 4; ++i) out[p * 4 + i] = 0.0;
             for (int j = 0; j < kQ; ++j) {
                 const double gj = hg[(size_t)j * P + p];
-                for (int i = 0; i < 4; ++i) grad[i] += gj * d[j * 4 + i];
+                for (int i = 0; i < 4; ++i) out[p * 4 + i] += gj * d[j * 4 + i];
             }
         }
+    }
+
+    void reverse_result(double grad[4]) override
+    {
+        std::vector<double> per((size_t)P * 4);
+        particle_gradients(per.data());
+        for (int i = 0; i < 4; ++i) grad[i] = 0.0;
+        for (int64_t p = 0; p < P; ++p)
+            for (int i = 0; i < 4; ++i) grad[i] += per[(size_t)p * 4 + i];
     }
 };
 
@@ -640,6 +651,14 @@ int mp_adjoint_reverse_result(mp_adjoint *a, double grad_theta[4])
     return guarded([&] {
         require(a && grad_theta, "handle and grad_theta are required");
         a->impl->reverse_result(grad_theta);
+    });
+}
+
+int mp_adjoint_particle_gradients(mp_adjoint *a, double *grad)
+{
+    return guarded([&] {
+        require(a && grad, "handle and grad are required");
+        a->impl->particle_gradients(grad);
     });
 }
 
