@@ -547,9 +547,8 @@ void split_host(const std::vector<double> &x, std::vector<__bf16> &hi, std::vect
 int replica_ld(int64_t n) { return 3 * pad_to(n, mp::kParticlePad); }
 int replica_rows(int64_t reps) { return pad_to(reps, mp::kBM); }
 
-void launch_product(int mode, const float *a32, const __bf16 *a_hi, const __bf16 *a_lo,
-                    const float *b32, const __bf16 *b_hi, const __bf16 *b_lo, float *c, int rows,
-                    int ld, hipStream_t s)
+void launch_product(int mode, const float *a32, const float *b32, const __bf16 *b_hi,
+                    const __bf16 *b_lo, float *c, int rows, int ld, hipStream_t s)
 {
     require(rows % mp::kBM == 0 && ld % mp::kBN == 0 && ld % mp::kGemmTile == 0 &&
                 ld % (6 * mp::kBK) == 0,
@@ -560,12 +559,13 @@ void launch_product(int mode, const float *a32, const __bf16 *a_hi, const __bf16
     } else {
         const int nrb = rows / mp::kBM, ncb = ld / mp::kBN;
         hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel, dim3((unsigned)(nrb * ncb)), dim3(512),
-                           0, s, a_hi, a_lo, b_hi, b_lo, c, ld, ld, nrb, ncb);
+                           0, s, a32, b_hi, b_lo, c, ld, ld, nrb, ncb);
     }
 }
 
 // All members are one cluster replicated: state [rows][ld] replica-major,
-// four launches per Heun step (product, stage 1, product, stage 2).
+// four launches per Heun step (product, stage 1, product, stage 2); the
+// products read the FP32 state and split it to BF16 themselves.
 struct ReplicaRun : GroupBase {
     int n = 0, reps = 0, ld = 0, rows = 0, mode = MP_DIPOLAR_GEMM_BF16X3, n_waves = 0;
     int shape = MP_FIELD_CONSTANT;
@@ -573,9 +573,9 @@ struct ReplicaRun : GroupBase {
     bool want_traj = false, want_ens = false;
     int64_t S = 0, s_next = 0;
     std::vector<int64_t> sched, slot_particle;
-    DevBuf<float> d_m, d_mt, d_delta, d_h, d_kex, d_key, d_kez, d_edx, d_edy, d_edz, d_sig, d_b32,
+    DevBuf<float> d_m, d_mt, d_h, d_kex, d_key, d_kez, d_edx, d_edy, d_edz, d_sig, d_b32,
         d_samples;
-    DevBuf<__bf16> d_mhi, d_mlo, d_bhi, d_blo;
+    DevBuf<__bf16> d_bhi, d_blo;
     DevBuf<uint32_t> d_key0, d_key1, d_status;
     DevBuf<double> d_msw, d_partial, d_ens;
     DevBuf<int64_t> d_sched;
@@ -653,14 +653,7 @@ struct ReplicaRun : GroupBase {
         n_done = std::min(sp.start_step, n_total);
         d_m.upload(cast_vec<float>(m0));
         d_mt.upload(std::vector<float>(m0.size(), 0.f));
-        d_delta.upload(std::vector<float>(m0.size(), 0.f));
         d_h.upload(std::vector<float>(m0.size(), 0.f));
-        d_mhi.alloc(m0.size());
-        d_mlo.alloc(m0.size());
-        hipLaunchKernelGGL(mp::split_kernel, dim3((unsigned)((m0.size() + 255) / 256)), dim3(256),
-                           0, 0, d_m.p, d_mhi.p, d_mlo.p, (int64_t)m0.size());
-        MP_HIP(hipGetLastError());
-        MP_HIP(hipDeviceSynchronize());
         d_status.upload(std::vector<uint32_t>((size_t)reps * n, 0));
         d_sched.upload(sched);
         const int threads = reps * n;
@@ -684,8 +677,7 @@ struct ReplicaRun : GroupBase {
         a.msw = d_msw.p;
         a.tag = sp.noise_tag;
         a.renorm = sp.renorm ? 1 : 0;
-        a.m = d_m.p; a.mt = d_mt.p; a.delta = d_delta.p; a.h = d_h.p;
-        a.m_hi = d_mhi.p; a.m_lo = d_mlo.p;
+        a.m = d_m.p; a.mt = d_mt.p; a.h = d_h.p;
         a.status = d_status.p;
         a.sched = d_sched.p;
         a.S = S;
@@ -705,11 +697,9 @@ struct ReplicaRun : GroupBase {
             const double t1 = (double)(st + 1) * dt;
             const float h1 = (float)applied_host(shape, happ, fapp, t1);
             const float h2 = (float)applied_host(shape, happ, fapp, t1 + dt);
-            launch_product(mode, d_m.p, d_mhi.p, d_mlo.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows,
-                           ld, s);
+            launch_product(mode, d_m.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows, ld, s);
             hipLaunchKernelGGL(mp::replica_stage1_kernel, grid, dim3(256), 0, s, a, st, h1);
-            launch_product(mode, d_mt.p, d_mhi.p, d_mlo.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows,
-                           ld, s);
+            launch_product(mode, d_mt.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows, ld, s);
             while (s_next < S && sched[(size_t)s_next] < st + 1) ++s_next;
             int n_rec = 0;
             while (s_next + n_rec < S && sched[(size_t)(s_next + n_rec)] == st + 1) ++n_rec;
@@ -1383,7 +1373,7 @@ int mp_dipolar_field(const double *location, const double *radius, const double 
             MP_HIP(hipMemcpy(out.data(), dc.p, out.size() * sizeof(double), hipMemcpyDeviceToHost));
         } else {
             DevBuf<float> a32, b32, c32((size_t)rows * ld);
-            DevBuf<__bf16> ahi, alo, bhi, blo;
+            DevBuf<__bf16> bhi, blo;
             a32.upload(cast_vec<float>(ma));
             if (mode == MP_DIPOLAR_GEMM_F32) {
                 b32.upload(cast_vec<float>(D));
@@ -1392,13 +1382,8 @@ int mp_dipolar_field(const double *location, const double *radius, const double 
                 split_host(D, hi, lo);
                 bhi.upload(hi);
                 blo.upload(lo);
-                ahi.alloc(ma.size());
-                alo.alloc(ma.size());
-                hipLaunchKernelGGL(mp::split_kernel, dim3((unsigned)((ma.size() + 255) / 256)),
-                                   dim3(256), 0, 0, a32.p, ahi.p, alo.p, (int64_t)ma.size());
-                MP_HIP(hipGetLastError());
             }
-            launch_product(mode, a32.p, ahi.p, alo.p, b32.p, bhi.p, blo.p, c32.p, rows, ld, 0);
+            launch_product(mode, a32.p, b32.p, bhi.p, blo.p, c32.p, rows, ld, 0);
             MP_HIP(hipGetLastError());
             std::vector<float> o32(out.size());
             MP_HIP(hipMemcpy(o32.data(), c32.p, o32.size() * sizeof(float), hipMemcpyDeviceToHost));

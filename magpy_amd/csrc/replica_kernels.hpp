@@ -61,7 +61,8 @@ __device__ inline void store_tile32(float *C, int ldc, int r0, int c0, const f32
     }
 }
 
-// Ht = Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T with LDS staging.  Block
+// Ht = Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T with LDS staging; Mt is
+// read in FP32 and split on its way to LDS, D comes pre-split.  Block
 // tile 128 rows x 192 columns; each tile of 64 x 96 (2 x 3 MFMA tiles) is
 // computed by two waves (8 waves per block), one per 16-k half of every
 // 32-k stage, so each SIMD holds two waves and one can issue MFMAs while the
@@ -74,8 +75,7 @@ __device__ inline void store_tile32(float *C, int ldc, int r0, int c0, const f32
 // dependent ones.
 constexpr int kBM = 128, kBN = 192, kBK = 32;
 
-static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(const __bf16 *__restrict__ a_hi,
-                                                                 const __bf16 *__restrict__ a_lo,
+static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(const float *__restrict__ a,
                                                                  const __bf16 *__restrict__ b_hi,
                                                                  const __bf16 *__restrict__ b_lo,
                                                                  float *__restrict__ c, int K,
@@ -104,21 +104,24 @@ static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(const __
 #pragma unroll
             for (int q = 0; q < 16; ++q) acc[t][u][q] = 0.f;
 
-    // staging: A 512 chunks of 16 B per part (one per thread), B 768 (threads
-    // 0..255 take a second one); three stages in flight
+    // staging: A is FP32, 1024 chunks of 4 values (two per thread), split
+    // into hi/lo BF16 on the way to LDS; B 768 chunks of 8 BF16 per part
+    // (threads 0..255 take a second one); three stages in flight
     typedef __attribute__((ext_vector_type(4))) uint32_t u32x4v;
-    u32x4v ga[3][2], gb[3][2][2];
+    typedef __attribute__((ext_vector_type(4))) float f32x4v;
+    typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4;
+    f32x4v ga[3][2];
+    u32x4v gb[3][2][2];
     const bool two_b = tid < 256;
     const int n_stages = K / kBK;
     auto load_stage = [&](auto set_c, int st) {
         constexpr int R = decltype(set_c)::value;
         if (st >= n_stages) return;
         const int k0 = st * kBK;
-        {
-            const int row = tid >> 2, kc = tid & 3;
-            const size_t off = (size_t)(row0 + row) * K + k0 + 8 * kc;
-            ga[R][0] = *reinterpret_cast<const u32x4v *>(a_hi + off);
-            ga[R][1] = *reinterpret_cast<const u32x4v *>(a_lo + off);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int ch = tid + 512 * i, row = ch >> 3, kc = ch & 7;
+            ga[R][i] = *reinterpret_cast<const f32x4v *>(a + (size_t)(row0 + row) * K + k0 + 4 * kc);
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -131,11 +134,19 @@ static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(const __
     };
     auto store_stage = [&](auto set_c, auto buf_c) {
         constexpr int R = decltype(set_c)::value, B = decltype(buf_c)::value;
-        {
-            const int row = tid >> 2, kc = tid & 3;
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
-                *reinterpret_cast<u32x4v *>(sa(B, h) + row * kLdsRow + 8 * kc) = ga[R][h];
+        for (int i = 0; i < 2; ++i) {
+            const int ch = tid + 512 * i, row = ch >> 3, kc = ch & 7;
+            bf16x4 hi, lo;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float x = ga[R][i][q];
+                const __bf16 h = (__bf16)x;
+                hi[q] = h;
+                lo[q] = (__bf16)(x - (float)h);
+            }
+            *reinterpret_cast<bf16x4 *>(sa(B, 0) + row * kLdsRow + 4 * kc) = hi;
+            *reinterpret_cast<bf16x4 *>(sa(B, 1) + row * kLdsRow + 4 * kc) = lo;
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -278,18 +289,6 @@ static __global__ void gemm_f64_kernel(const double *__restrict__ a, const doubl
     c[(size_t)r * cols + col] = s;
 }
 
-// FP32 -> (hi, lo) BF16 split of n values.
-static __global__ void split_kernel(const float *__restrict__ x, __bf16 *__restrict__ hi,
-                             __bf16 *__restrict__ lo, int64_t n)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    __bf16 h, l;
-    split_bf16(x[i], h, l);
-    hi[i] = h;
-    lo[i] = l;
-}
-
 // ------------------------------------------------------------------ Heun stages
 // Per-replica Heun stages around the product.  Thread = (replica r, particle
 // p), p fastest; state rows are replica-major with leading dimension ld
@@ -307,8 +306,7 @@ struct ReplicaArgs {
     const double *msw;             // per replica Ms
     uint32_t tag;
     int renorm;
-    float *m, *mt, *delta, *h;     // [reps][ld]: state, predictor, stage-1 increment, field
-    __bf16 *m_hi, *m_lo;           // split of the state the next product reads
+    float *m, *mt, *h;             // [reps][ld]: state, predictor, field
     uint32_t *status;              // [reps * n]
     // sampling
     const int64_t *sched;
@@ -341,7 +339,7 @@ __device__ inline void replica_normals(const ReplicaArgs &a, int r, int p, int64
     sz = sig * w2;
 }
 
-// stage 1: m~ = m + L(m, h dt + s), stores m~, the increment and the split of m~
+// stage 1: m~ = m + L(m, h dt + s), stores m~ (the next product's operand)
 static __global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs a, int64_t n_step, float h1)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -358,15 +356,13 @@ static __global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs 
                 vz = d * a.edz[p] + ((hz + h1) * dt + sz);
     float ax, ay, az;
     llg_map(mx, my, mz, vx, vy, vz, a.alpha, ax, ay, az);
-    const float tx = mx + ax, ty = my + ay, tz = mz + az;
-    a.delta[ox] = ax; a.delta[oy] = ay; a.delta[oz] = az;
-    a.mt[ox] = tx; a.mt[oy] = ty; a.mt[oz] = tz;
-    split_bf16(tx, a.m_hi[ox], a.m_lo[ox]);
-    split_bf16(ty, a.m_hi[oy], a.m_lo[oy]);
-    split_bf16(tz, a.m_hi[oz], a.m_lo[oz]);
+    a.mt[ox] = mx + ax;
+    a.mt[oy] = my + ay;
+    a.mt[oz] = mz + az;
 }
 
-// stage 2: m' = m + (L(m, v1) + L(m~, v2)) / 2, stores m', its split, samples
+// stage 2: m' = m + (L(m, v1) + L(m~, v2)) / 2 with L(m, v1) = m~ - m;
+// stores m' and the samples
 static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs a, int64_t n_step, float h2,
                                                              int64_t s_first, int n_rec)
 {
@@ -385,9 +381,10 @@ static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs 
                 vz = d * a.edz[p] + ((hz + h2) * dt + sz);
     float bx, by, bz;
     llg_map(tx, ty, tz, vx, vy, vz, a.alpha, bx, by, bz);
-    float mx = a.m[ox] + 0.5f * (a.delta[ox] + bx);
-    float my = a.m[oy] + 0.5f * (a.delta[oy] + by);
-    float mz = a.m[oz] + 0.5f * (a.delta[oz] + bz);
+    const float m0x = a.m[ox], m0y = a.m[oy], m0z = a.m[oz];
+    float mx = m0x + 0.5f * ((tx - m0x) + bx);
+    float my = m0y + 0.5f * ((ty - m0y) + by);
+    float mz = m0z + 0.5f * ((tz - m0z) + bz);
     if (a.renorm) {
         const float nrm = sqrtf(mx * mx + my * my + mz * mz);
         mx = mx / nrm;
@@ -396,9 +393,6 @@ static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs 
     }
     if (live) {
         a.m[ox] = mx; a.m[oy] = my; a.m[oz] = mz;
-        split_bf16(mx, a.m_hi[ox], a.m_lo[ox]);
-        split_bf16(my, a.m_hi[oy], a.m_lo[oy]);
-        split_bf16(mz, a.m_hi[oz], a.m_lo[oz]);
         if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[i] |= 1u;
     }
     // samples s_first .. s_first + n_rec - 1 are all recorded after this step

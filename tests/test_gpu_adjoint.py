@@ -182,3 +182,23 @@ def test_value_only_forward_has_no_reverse():
     r.forward(THETA, record_gradient=False)
     with pytest.raises(ValueError):
         r.reverse([0.0, 1.0, 1.0], 0.0)
+
+
+def test_particle_gradients_split_the_total():
+    print("CASE adjoint_particle_split", flush=True)
+    shape, ax, seeds = problem(96, 9)
+    ks, N = [0, 150, 300], 300
+    r = runner(shape, ax, seeds, 1e10, N, ks, 100, "fp64")
+    r.forward(THETA)
+    w = np.array([0.0, 2.0, -1.0])
+    total = r.reverse(w, 0.25)
+    per = r.particle_gradients()
+    assert np.allclose(per.sum(0), total, rtol=1e-12, atol=0)
+    # one particle's part = the oracle tangent of that particle alone
+    p = 17
+    q, dq = core.adjoint_reduced(THETA, shape[p:p + 1], MS, T, DT)
+    pr = O.path_problem(ax[p:p + 1], ax[p:p + 1], q[:, 0], q[:, 1], q[:, 2], q[:, 3], 1e10 * DT,
+                        seeds[p:p + 1], N, ks)
+    tan, tan_f = O.path_tangent(pr, np.ascontiguousarray(np.transpose(dq, (0, 2, 1))))
+    ref = (MS / 96) * (w @ tan) + (0.25 / 96) * tan_f
+    assert np.all(np.abs(per[p] - ref) <= 1e-9 * np.abs(ref)), (per[p], ref)
