@@ -57,7 +57,7 @@ FLOP_PER_PAIR_STEP = 48               # 24 FLOP per ordered pair, two field eval
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (published)
 # HBM bytes of one default launch (65,536 particles x 131,072 steps), from
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) scaled against a 1 GiB
-# device copy; see profiles/heun_indep_fp32_65536_ac_field.md.
+# device copy; see profiles/heun_indep_pc_fp32_65536_ac_field.md.
 TRAFFIC_BYTES_DEFAULT_LAUNCH = 13.30e6
 DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256),
             "replicas": (4096, 64), "adjoint": (16384, 1 << 20)}
@@ -288,15 +288,16 @@ def adjoint_bench(args):
     os.environ["MAGPY_AMD_ADJOINT_STEPS_PER_LAUNCH"] = str(B)
     run = core.Adjoint(shape, ax, ax, inp["seeds"], 400e3, 300.0, 3e5, 1e-14, N, ks, 1000, "fp32")
     run.begin(theta, True)
-    t_fwd = 0.0
-    for b in range(n_blocks):
-        if b == args.warmup:
-            info0 = run.info()
-        t0 = time.perf_counter()
+    for b in range(args.warmup):
         run.forward_advance(B)
-        if b >= args.warmup:
-            t_fwd += time.perf_counter() - t0
-    info1 = run.info()
+    info0 = run.info()   # waits for the warmup blocks' maps
+    t0 = time.perf_counter()
+    for b in range(args.steps):
+        # returns when the block's forward kernel is done; its maps overlap
+        # the next block's forward kernel
+        run.forward_advance(B)
+    info1 = run.info()   # waits for the last block's maps
+    t_fwd = time.perf_counter() - t0
     ens, fin, status = run.forward_result()
     w = np.linspace(-1.0, 1.0, len(ks))
     t0 = time.perf_counter()

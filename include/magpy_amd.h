@@ -244,7 +244,9 @@ void mp_adjoint_destroy(mp_adjoint *a);
 /* start a forward pass at theta (drops any earlier pass); record_gradient
  * = 0 runs the forward pass only (values, e.g. for finite differences) */
 int mp_adjoint_begin(mp_adjoint *a, const double theta[4], int record_gradient);
-/* advance the forward pass by up to max_steps; *steps_done = steps so far */
+/* advance the forward pass by up to max_steps; *steps_done = steps so far.
+ * Returns when the forward kernels are done; the adjoint maps of the last
+ * block may still run and overlap the next call's forward kernels */
 int mp_adjoint_forward_advance(mp_adjoint *a, int64_t max_steps, int64_t *steps_done);
 /* finish the forward pass and return its outputs (ens_mz [S], final_mz);
  * *status (optional) = OR over particles of MP_STATUS_NONFINITE */
@@ -260,8 +262,8 @@ int mp_adjoint_reverse_result(mp_adjoint *a, double grad_theta[4]);
  * that flows through each particle's trajectory (they sum to grad_theta) */
 int mp_adjoint_particle_gradients(mp_adjoint *a, double *grad);
 /* device milliseconds of the forward kernels and of the gradient kernels
- * (transfers, folding, scan) so far, and the device bytes held for
- * checkpoints and segment scratch */
+ * (transfers, folding, scan) so far (waits for queued work), and the device
+ * bytes held for checkpoints and segment scratch */
 int mp_adjoint_info(mp_adjoint *a, double *forward_ms, double *reverse_ms, int64_t *checkpoint_bytes,
                     int64_t *scratch_bytes);
 /* Reduced per-particle parameters q[p] = (dt, sr sqrt(dt), alpha, happ) at

@@ -100,6 +100,16 @@ int64_t env_steps_per_launch()
     return 1 << 16;
 }
 
+// lanes (segment x particle) per adjoint-map launch
+int64_t env_batch_lanes()
+{
+    if (const char *env = std::getenv("MAGPY_AMD_ADJOINT_BATCH_LANES")) {
+        long long v = std::atoll(env);
+        if (v > 0) return (int64_t)v;
+    }
+    return 1 << 18;
+}
+
 struct AdjointBase {
     virtual ~AdjointBase() = default;
     virtual void begin(const double th[4], bool record) = 0;
@@ -109,7 +119,9 @@ struct AdjointBase {
     virtual int64_t reverse_advance(int64_t n) = 0;
     virtual void reverse_result(double grad[4]) = 0;
     virtual void particle_gradients(double *out) = 0;
+    virtual void drain() = 0;
     double fwd_ms = 0, rev_ms = 0;
+    int device = 0;
     int64_t ck_bytes = 0, scratch_bytes = 0;
 };
 
@@ -161,7 +173,6 @@ struct AdjointRun final : AdjointBase {
     std::vector<double> shape, m0;
     std::vector<int64_t> sched;
     PieceTable pt;
-    int device;
     int64_t P, N, C, S, n_ck, spl, segs_per_block, batch_segs, max_batch_pieces;
     int n_waves;
     // forward kernels on sg.s, the adjoint maps on tr.s: the two run
@@ -182,8 +193,9 @@ struct AdjointRun final : AdjointBase {
     int64_t fwd_done = 0;
     double lambda_z0 = 0.0;
 
-    explicit AdjointRun(const mp_adjoint_spec &s) : sp(s), device(s.device)
+    explicit AdjointRun(const mp_adjoint_spec &s) : sp(s)
     {
+        device = s.device;
         P = sp.n_particles;
         N = sp.n_steps;
         C = sp.checkpoint_interval;
@@ -201,12 +213,16 @@ struct AdjointRun final : AdjointBase {
         sp.seed = nullptr;
         pt.build(sched, N, C);
         n_waves = (int)((P + mp::kWave - 1) / mp::kWave);
-        // transfer batches: about 2^18 lanes, scratch at most 4 GiB
+        // transfer batches: about 2^18 lanes (4,096 waves; 2^18 to 2^21
+        // measure the same), scratch at most 8 GiB, the block's segments
+        // split evenly
         const int64_t seg_len = std::min(C, N);
-        const int64_t by_lanes = std::max<int64_t>(1, (1 << 18) / P);
-        const int64_t by_bytes =
-            std::max<int64_t>(1, ((int64_t)4 << 30) / (seg_len * 3 * P * (int64_t)sizeof(T)));
-        batch_segs = std::min({segs_per_block, by_lanes, by_bytes});
+        const int64_t by_lanes = std::max<int64_t>(1, env_batch_lanes() / P);
+        const int64_t by_bytes = std::max<int64_t>(
+            1, ((int64_t)8 << 30) / (seg_len * mp::kScratchRows * P * (int64_t)sizeof(T)));
+        const int64_t most = std::min({segs_per_block, by_lanes, by_bytes});
+        const int64_t n_batches = (segs_per_block + most - 1) / most;
+        batch_segs = (segs_per_block + n_batches - 1) / n_batches;
         max_batch_pieces = 1;
         for (int64_t b0 = 0; b0 < n_ck; b0 += segs_per_block) {
             const int64_t b1 = std::min(n_ck, b0 + segs_per_block);
@@ -248,7 +264,7 @@ struct AdjointRun final : AdjointBase {
         }
         partial.alloc((size_t)(S + 1) * n_waves);
         if (S) d_sched.upload(sched);
-        ftab.alloc((size_t)spl + 2);
+        ftab.alloc((size_t)spl + 5);  // the last 4-step group reads 4 past n1 + 1
         ftab_tr.alloc((size_t)spl + 2);
         d_seg_first.upload(pt.seg_first);
         d_seg_count.upload(pt.seg_count);
@@ -262,6 +278,10 @@ struct AdjointRun final : AdjointBase {
     {
         (void)hipStreamSynchronize(sg.s);
         (void)hipStreamSynchronize(tr.s);
+        for (Timing &t : timings) {
+            (void)hipEventDestroy(t.e0);
+            (void)hipEventDestroy(t.e1);
+        }
         for (int b = 0; b < 2; ++b) {
             if (ev_fwd[b]) (void)hipEventDestroy(ev_fwd[b]);
             if (ev_tr[b]) (void)hipEventDestroy(ev_tr[b]);
@@ -273,7 +293,7 @@ struct AdjointRun final : AdjointBase {
     {
         if (scratch.p) return;
         const int64_t seg_len = std::min(C, N);
-        scratch.alloc((size_t)seg_len * 3 * (size_t)(batch_segs * P));
+        scratch.alloc((size_t)seg_len * mp::kScratchRows * (size_t)(batch_segs * P));
         scratch_bytes = (int64_t)(scratch.n * sizeof(T));
         pieces.alloc((size_t)max_batch_pieces * mp::kTransfer * P);
         acc.alloc((size_t)mp::kTransfer * P);
@@ -337,25 +357,40 @@ struct AdjointRun final : AdjointBase {
         MP_HIP(hipEventRecord(t.e1, st));
     }
 
-    // wait for both streams and account the timed launches
-    void drain()
+    // account the timed launches that have finished
+    void collect()
     {
-        MP_HIP(hipStreamSynchronize(sg.s));
-        MP_HIP(hipStreamSynchronize(tr.s));
+        size_t keep = 0;
         for (Timing &t : timings) {
+            const hipError_t q = hipEventQuery(t.e1);
+            if (q == hipErrorNotReady) {
+                timings[keep++] = t;
+                continue;
+            }
+            MP_HIP(q);
             float ms = 0.f;
             MP_HIP(hipEventElapsedTime(&ms, t.e0, t.e1));
             *t.acc += ms;
             (void)hipEventDestroy(t.e0);
             (void)hipEventDestroy(t.e1);
         }
-        timings.clear();
+        timings.resize(keep);
+    }
+
+    // wait for both streams and account every timed launch
+    void drain() override
+    {
+        MP_HIP(hipStreamSynchronize(sg.s));
+        MP_HIP(hipStreamSynchronize(tr.s));
+        collect();
     }
 
     void begin(const double th[4], bool record) override
     {
         reduce_params(sp, th, q, dq);
         MP_HIP(hipSetDevice(device));
+        // the previous pass's adjoint maps may still read sig and the state
+        drain();
         std::vector<T> hs(P), hx(P), hy(P), hz(P);
         for (int64_t p = 0; p < P; ++p) {
             hs[p] = (T)q[p * kQ + 1];
@@ -367,7 +402,6 @@ struct AdjointRun final : AdjointBase {
         MP_HIP(hipMemcpy(mx.p, hx.data(), P * sizeof(T), hipMemcpyHostToDevice));
         MP_HIP(hipMemcpy(my.p, hy.data(), P * sizeof(T), hipMemcpyHostToDevice));
         MP_HIP(hipMemcpy(mz.p, hz.data(), P * sizeof(T), hipMemcpyHostToDevice));
-        drain();
         // cleared on the forward stream, ahead of its kernels
         MP_HIP(hipMemsetAsync(status.p, 0, P * sizeof(uint32_t), sg.s));
         MP_HIP(hipMemsetAsync(partial.p, 0, partial.n * sizeof(double), sg.s));
@@ -448,7 +482,7 @@ struct AdjointRun final : AdjointBase {
             // the maps of block k - 2 must be done with this checkpoint buffer
             if (tr_pending[buf]) MP_HIP(hipStreamWaitEvent(sg.s, ev_tr[buf], 0));
             timed(fwd_ms, sg.s, [&] {
-                table(ftab, sg.s, n0, n1 - n0 + 2);
+                table(ftab, sg.s, n0, n1 - n0 + 5);
                 a.tab0 = n0;
                 a.n0 = n0;
                 a.n1 = n1;
@@ -456,7 +490,7 @@ struct AdjointRun final : AdjointBase {
                 a.ck_base = j0;
                 a.s_next = std::lower_bound(sched.begin(), sched.end(), n0) - sched.begin();
                 hipLaunchKernelGGL(mp::path_forward_kernel<T>, dim3((unsigned)n_waves),
-                                   dim3(mp::kWave), 0, sg.s, a);
+                                   dim3(2 * mp::kWave), 0, sg.s, a);
                 MP_HIP(hipGetLastError());
             });
             MP_HIP(hipEventRecord(ev_fwd[buf], sg.s));
@@ -471,7 +505,10 @@ struct AdjointRun final : AdjointBase {
             }
             fwd_done = n1;
         }
-        drain();
+        // the forward stream is done; the maps of the last block may still
+        // run on the transfer stream, overlapping the next call's forward
+        MP_HIP(hipStreamSynchronize(sg.s));
+        collect();
         if (fwd_done == N) fwd_finished = true;
         return fwd_done;
     }
@@ -667,6 +704,8 @@ int mp_adjoint_info(mp_adjoint *a, double *forward_ms, double *reverse_ms,
 {
     return guarded([&] {
         require(a != nullptr, "handle is required");
+        MP_HIP(hipSetDevice(a->impl->device));
+        a->impl->drain();
         if (forward_ms) *forward_ms = a->impl->fwd_ms;
         if (reverse_ms) *reverse_ms = a->impl->rev_ms;
         if (checkpoint_bytes) *checkpoint_bytes = a->impl->ck_bytes;

@@ -17,8 +17,8 @@
 // map of its end value.  Those maps (transfers, below) are computed right
 // after each forward batch, for all segments of the batch in parallel: a
 // lane per (segment, particle) recomputes its segment from the checkpoint
-// into scratch ([C][3][lanes], read back only by the lane that wrote it) and
-// carries three basis adjoints back through it.  They are folded into one
+// into scratch ([C][6][lanes]: states and normals, read back only by the lane
+// that wrote it) and carries three basis adjoints back through it.  They are folded into one
 // transfer per sampling interval, and a reverse pass with given weights is
 // then a short scan over the intervals.
 //
@@ -87,14 +87,16 @@ __device__ inline V3<T> group_w(const T z[12], int i)
     return V3<T>{z[9], z[10], z[11]};
 }
 
-// field h(m, s) and v = dt h + sig w
+// field h(m, s) and v = dt h + sig w (fused multiply-adds fixed by the
+// source, as in llg_map, so the recompute reproduces the forward pass)
 template <typename T>
 __device__ inline V3<T> path_v(const V3<T> &m, const V3<T> &e, const V3<T> &w, T dt, T sig,
                                T happ, T s)
 {
-    const T me = dot(m, e);
-    return V3<T>{dt * (me * e.x) + sig * w.x, dt * (me * e.y) + sig * w.y,
-                 dt * (me * e.z + happ * s) + sig * w.z};
+#pragma clang fp contract(off)
+    const T me = fmad(m.x, e.x, fmad(m.y, e.y, m.z * e.z));
+    return V3<T>{fmad(dt, me * e.x, sig * w.x), fmad(dt, me * e.y, sig * w.y),
+                 fmad(dt, fmad(me, e.z, happ * s), sig * w.z)};
 }
 
 template <typename T>
@@ -109,11 +111,12 @@ template <typename T>
 __device__ inline V3<T> path_step(const V3<T> &m, const V3<T> &e, const V3<T> &w, T dt, T sig,
                                   T al, T happ, T s1, T s2)
 {
+#pragma clang fp contract(off)
     const V3<T> a1 = path_L(m, path_v(m, e, w, dt, sig, happ, s1), al);
     const V3<T> mt{m.x + a1.x, m.y + a1.y, m.z + a1.z};
     const V3<T> a2 = path_L(mt, path_v(mt, e, w, dt, sig, happ, s2), al);
-    return V3<T>{m.x + T(0.5) * (a1.x + a2.x), m.y + T(0.5) * (a1.y + a2.y),
-                 m.z + T(0.5) * (a1.z + a2.z)};
+    return V3<T>{fmad(T(0.5), a1.x + a2.x, m.x), fmad(T(0.5), a1.y + a2.y, m.y),
+                 fmad(T(0.5), a1.z + a2.z, m.z)};
 }
 
 // Unscaled Wiener normals of step group g (steps 4g..4g+3): w of step 4g+i
@@ -138,56 +141,116 @@ __device__ inline void store3(T *base, size_t stride, int p, const V3<T> &v)
     base[2 * stride + p] = v.z;
 }
 
+// wave sum of m_z into row `row` of the partial sums; `wave` indexes the
+// 64 particles of the calling wave
 template <typename T>
-__device__ inline void path_record(const PathArgs<T> &a, int64_t row, bool live, T mz)
+__device__ inline void path_record(const PathArgs<T> &a, int64_t row, int wave, bool live, T mz)
 {
     double v = live ? (double)mz : 0.0;
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-        const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-        a.mz_partial[(size_t)row * a.n_waves + wave] = v;
-    }
+    if ((threadIdx.x & (kWave - 1)) == 0) a.mz_partial[(size_t)row * a.n_waves + wave] = v;
 }
 
-// Forward block [n0, n1): checkpoints at multiples of C, wave sums of m_z at
-// the sample steps, the final row after step N.  n0 is a multiple of 4.
+// Step groups per half of the forward kernel's normal ring
 template <typename T>
-__global__ __launch_bounds__(kWave) void path_forward_kernel(PathArgs<T> a)
+struct FwdRing {
+    static constexpr int kGroups = sizeof(T) == 4 ? 8 : 4;
+};
+
+// Forward block [n0, n1): checkpoints at multiples of C, wave sums of m_z at
+// the sample steps, the final row after step N.  n0 is a multiple of 4 (and
+// of C).  A workgroup of two waves serves 64 particles: wave 1 generates the
+// Wiener normals of 4-step groups into an LDS ring of two halves, wave 0
+// integrates from the other half, and the two swap halves at a barrier every
+// kGroups groups.  The integrating wave thus carries only the Heun steps; it
+// reads the group's five field values once and keeps the next sample step in
+// a register.
+template <typename T>
+__global__ __launch_bounds__(2 * kWave) void path_forward_kernel(PathArgs<T> a)
 {
-    const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int K = FwdRing<T>::kGroups;
+    __shared__ T ring[2][K][12][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool producer = __builtin_amdgcn_readfirstlane(threadIdx.x) >= kWave;
+    const int wave = blockIdx.x;
+    const int p0 = wave * kWave + lane;
     const bool live = p0 < a.n;
     const int p = live ? p0 : a.n - 1;
     const size_t P = (size_t)a.n;
+    const int64_t g0 = a.n0 >> 2, g1 = (a.n1 + 3) >> 2;
+    const int64_t n_chunks = (g1 - g0 + K - 1) / K;
+    PhiloxKeys keys;
+    keys.init(a.key0[p], a.key1[p]);
     V3<T> m{a.mx[p], a.my[p], a.mz[p]};
     const V3<T> e{a.ex[p], a.ey[p], a.ez[p]};
     const T sig = a.sig[p], dt = a.dt, al = a.alpha, ha = a.happ;
-    PhiloxKeys keys;
-    keys.init(a.key0[p], a.key1[p]);
     int64_t s = a.s_next;
-    // next checkpoint step and its index (n0 need not be a checkpoint)
-    int64_t ck_i = (a.n0 + a.ck_every - 1) / a.ck_every;
-    int64_t ck_n = ck_i * a.ck_every;
-    T z[12];
-    for (int64_t n = a.n0; n < a.n1; ++n) {
-        if (n == ck_n) {
-            if (live) store3(a.ck + (size_t)(ck_i - a.ck_base) * 3 * P, P, p, m);
-            ++ck_i;
-            ck_n += a.ck_every;
-        }
-        while (s < a.S && a.sched[s] == n) path_record(a, s++, live, m.z);
-        const int i = (int)(n & 3);
-        if (i == 0 || n == a.n0) path_group_normals<T>(n >> 2, a.tag, keys, z);
-        T w0 = z[0], w1 = z[1], w2 = z[2];
-        if (i == 1) { w0 = z[3]; w1 = z[4]; w2 = z[5]; }
-        else if (i == 2) { w0 = z[6]; w1 = z[7]; w2 = z[8]; }
-        else if (i == 3) { w0 = z[9]; w1 = z[10]; w2 = z[11]; }
-        const T s1 = a.ftab[n + 1 - a.tab0], s2 = a.ftab[n + 2 - a.tab0];
-        m = path_step(m, e, V3<T>{w0, w1, w2}, dt, sig, al, ha, s1, s2);
+    int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+    int64_t ck_n = (a.n0 + a.ck_every - 1) / a.ck_every * a.ck_every;
+    if (!producer) {
+        // the integrating wave takes issue priority over co-resident waves
+        __builtin_amdgcn_s_setprio(3);
     }
+    // pass c: the producer fills chunk c, the integrator runs chunk c - 1
+    for (int64_t c = 0; c <= n_chunks; ++c) {
+        if (producer) {
+            if (c < n_chunks) {
+                T (*half)[12][kWave] = ring[c & 1];
+                for (int k = 0; k < K; ++k) {
+                    const int64_t g = g0 + c * K + k;
+                    if (g >= g1) break;
+                    T z[12];
+                    path_group_normals<T>(g, a.tag, keys, z);
+#pragma unroll
+                    for (int q = 0; q < 12; ++q) half[k][q][lane] = z[q];
+                }
+            }
+        } else if (c > 0) {
+            T (*half)[12][kWave] = ring[(c - 1) & 1];
+            for (int k = 0; k < K; ++k) {
+                const int64_t g = g0 + (c - 1) * K + k;
+                if (g >= g1) break;
+                T zc[12];
+#pragma unroll
+                for (int q = 0; q < 12; ++q) zc[q] = half[k][q][lane];
+                const int64_t nb = 4 * g;
+                const T *tab = a.ftab + (nb + 1 - a.tab0);
+                const T f0 = tab[0], f1 = tab[1], f2 = tab[2], f3 = tab[3], f4 = tab[4];
+                if (nb == ck_n) {
+                    if (live) store3(a.ck + (size_t)(nb / a.ck_every - a.ck_base) * 3 * P, P, p, m);
+                    ck_n += a.ck_every;
+                }
+                if (nb + 4 <= a.n1 && next_rec >= nb + 4) {
+                    m = path_step(m, e, V3<T>{zc[0], zc[1], zc[2]}, dt, sig, al, ha, f0, f1);
+                    m = path_step(m, e, V3<T>{zc[3], zc[4], zc[5]}, dt, sig, al, ha, f1, f2);
+                    m = path_step(m, e, V3<T>{zc[6], zc[7], zc[8]}, dt, sig, al, ha, f2, f3);
+                    m = path_step(m, e, V3<T>{zc[9], zc[10], zc[11]}, dt, sig, al, ha, f3, f4);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int64_t n = nb + i;
+                        if (n >= a.n1) break;
+                        while (n == next_rec) {
+                            path_record(a, s++, wave, live, m.z);
+                            next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+                        }
+                        const T sa = i == 0 ? f0 : i == 1 ? f1 : i == 2 ? f2 : f3;
+                        const T sb = i == 0 ? f1 : i == 1 ? f2 : i == 2 ? f3 : f4;
+                        m = path_step(m, e, group_w(zc, i), dt, sig, al, ha, sa, sb);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (producer) return;
     if (a.n1 == a.n_steps) {
-        while (s < a.S && a.sched[s] == a.n_steps) path_record(a, s++, live, m.z);
-        path_record(a, a.S, live, m.z);
+        while (next_rec == a.n_steps) {
+            path_record(a, s++, wave, live, m.z);
+            next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+        }
+        path_record(a, a.S, wave, live, m.z);
     }
     if (live) {
         a.mx[p] = m.x;
@@ -222,6 +285,7 @@ template <typename T>
 __device__ inline StepFwd<T> step_forward(const V3<T> &m, const V3<T> &e, const V3<T> &w, T dt,
                                           T sig, T al, T ha, T s1, T s2)
 {
+#pragma clang fp contract(off)
     StepFwd<T> f;
     f.m = m;
     f.w = w;
@@ -238,7 +302,7 @@ __device__ inline StepFwd<T> step_forward(const V3<T> &m, const V3<T> &e, const 
 // alpha, happ))^T lambda_{n+1}, for m_{n+1} = m + (L(m, v1) + L(mt, v2)) / 2.
 template <typename T>
 __device__ inline V3<T> step_adjoint(const StepFwd<T> &f, const V3<T> &e, T dt, T al, T ha,
-                                     const V3<T> &lam, double g[4])
+                                     const V3<T> &lam, T g[4])
 {
     const V3<T> &mm = f.m, &v1 = f.v1, &mt = f.mt, &v2 = f.v2, &w = f.w;
     const V3<T> ab{T(0.5) * lam.x, T(0.5) * lam.y, T(0.5) * lam.z};
@@ -251,16 +315,16 @@ __device__ inline V3<T> step_adjoint(const StepFwd<T> &f, const V3<T> &e, T dt, 
               -vx2.z + al * (T(2) * mt.z * v2b - tv2 * ab.z - v2.z * tb)};
     const V3<T> v2bar{tx2.x + al * (tt * ab.x - mt.x * tb), tx2.y + al * (tt * ab.y - mt.y * tb),
                       tx2.z + al * (tt * ab.z - mt.z * tb)};
-    g[2] += (double)(v2b * tt - tb * tv2);
+    g[2] += v2b * tt - tb * tv2;
     // v2 = dt h(mt, s2) + sig w
     const T ev2 = dot(e, v2bar);
     const T mte = dot(mt, e);
     mtb.x += dt * e.x * ev2;
     mtb.y += dt * e.y * ev2;
     mtb.z += dt * e.z * ev2;
-    g[0] += (double)(mte * ev2 + ha * f.s2 * v2bar.z);
-    g[1] += (double)dot(w, v2bar);
-    g[3] += (double)(dt * f.s2 * v2bar.z);
+    g[0] += mte * ev2 + ha * f.s2 * v2bar.z;
+    g[1] += dot(w, v2bar);
+    g[3] += dt * f.s2 * v2bar.z;
     // mt = m + a1
     const V3<T> a1b{ab.x + mtb.x, ab.y + mtb.y, ab.z + mtb.z};
     V3<T> mb{lam.x + mtb.x, lam.y + mtb.y, lam.z + mtb.z};
@@ -273,16 +337,16 @@ __device__ inline V3<T> step_adjoint(const StepFwd<T> &f, const V3<T> &e, T dt, 
     mb.z += -vx1.z + al * (T(2) * mm.z * v1b - mv1 * a1b.z - v1.z * mb1);
     const V3<T> v1bar{mx1.x + al * (mm2 * a1b.x - mm.x * mb1), mx1.y + al * (mm2 * a1b.y - mm.y * mb1),
                       mx1.z + al * (mm2 * a1b.z - mm.z * mb1)};
-    g[2] += (double)(v1b * mm2 - mb1 * mv1);
+    g[2] += v1b * mm2 - mb1 * mv1;
     // v1 = dt h(m, s1) + sig w
     const T ev1 = dot(e, v1bar);
     const T me = dot(mm, e);
     mb.x += dt * e.x * ev1;
     mb.y += dt * e.y * ev1;
     mb.z += dt * e.z * ev1;
-    g[0] += (double)(me * ev1 + ha * f.s1 * v1bar.z);
-    g[1] += (double)dot(w, v1bar);
-    g[3] += (double)(dt * f.s1 * v1bar.z);
+    g[0] += me * ev1 + ha * f.s1 * v1bar.z;
+    g[1] += dot(w, v1bar);
+    g[3] += dt * f.s1 * v1bar.z;
     return mb;
 }
 
@@ -298,12 +362,16 @@ struct TransferArgs {
     int64_t n_steps, ck_every;
     const T *ck;                    // checkpoints of this batch's segments [seg][3][n]
     int64_t seg0, n_segs;           // global index of the first segment, segments in batch
-    T *scratch;                     // [C][3][n_segs * n]
+    T *scratch;                     // [C][kScratchRows][n_segs * n]
     const int32_t *seg_first_piece, *seg_n_pieces;   // per segment of the batch
     const int64_t *piece_start;     // first step of each piece (run-wide index)
     int64_t piece_base;             // run-wide index of the batch's first piece
     double *pieces;                 // [piece - piece_base][21][n]
 };
+
+// Rows of scratch per step: the state m_n (3), then the unscaled normals w_n
+// (3), so that the reverse sweep needs no second pass of the generator.
+constexpr int kScratchRows = 6;
 
 // One lane per (segment, particle), particle fastest.
 template <typename T>
@@ -323,19 +391,40 @@ __global__ __launch_bounds__(kWave) void path_transfer_kernel(TransferArgs<T> a)
     PhiloxKeys keys;
     keys.init(a.key0[p], a.key1[p]);
     T z[12];
-    // 1. the segment's states m_{n_a} .. m_{n_b - 1}
+    // 1. the segment's states m_{n_a} .. m_{n_b - 1} and their normals
+    //    (n_a is a multiple of 4)
     V3<T> m = load3(a.ck + (size_t)sl * 3 * P, P, p);
     for (int64_t n = n_a; n < n_b; ++n) {
         const int i = (int)(n & 3);
         if (i == 0) path_group_normals<T>(n >> 2, a.tag, keys, z);
         const V3<T> w = group_w(z, i);
-        if (live) store3(a.scratch + (size_t)(n - n_a) * 3 * L, L, (int)gid, m);
+        if (live) {
+            T *row = a.scratch + (size_t)(n - n_a) * kScratchRows * L;
+            store3(row, L, (int)gid, m);
+            store3(row + 3 * L, L, (int)gid, w);
+        }
         m = path_step(m, e, w, dt, sig, al, ha, a.ftab[n + 1 - a.tab0], a.ftab[n + 2 - a.tab0]);
     }
     // 2. three adjoint channels (lambda = e_x, e_y, e_z at the end of each
-    //    piece) back through the segment; a piece ends at every sample step
+    //    piece) back through the segment; a piece ends at every sample step.
+    //    Parameter terms gather in gs (the state type) and move to g (double)
+    //    every 4 steps and at piece ends.
     V3<T> lam[3];
     double g[3][4];
+    T gs[3][4];
+    auto clear_gs = [&]() {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) gs[c][q] = T(0);
+    };
+    auto flush = [&]() {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) g[c][q] += (double)gs[c][q];
+        clear_gs();
+    };
     auto reset = [&]() {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -343,23 +432,28 @@ __global__ __launch_bounds__(kWave) void path_transfer_kernel(TransferArgs<T> a)
 #pragma unroll
             for (int q = 0; q < 4; ++q) g[c][q] = 0.0;
         }
+        clear_gs();
     };
     reset();
     int piece = a.seg_first_piece[sl] + a.seg_n_pieces[sl] - 1;
     int64_t piece_begin = a.piece_start[piece];
-    int64_t have = -1;
+    // the rows of step n - 1 load while step n runs
+    const T *row = a.scratch + (size_t)(n_b - 1 - n_a) * kScratchRows * L;
+    V3<T> mm_next = load3(row, L, (int)gid), w_next = load3(row + 3 * L, L, (int)gid);
     for (int64_t n = n_b - 1; n >= n_a; --n) {
-        if ((n >> 2) != have) {
-            have = n >> 2;
-            path_group_normals<T>(have, a.tag, keys, z);
+        const V3<T> mm = mm_next, w = w_next;
+        if (n > n_a) {
+            row -= kScratchRows * L;
+            mm_next = load3(row, L, (int)gid);
+            w_next = load3(row + 3 * L, L, (int)gid);
         }
-        const V3<T> w = group_w(z, (int)(n & 3));
-        const V3<T> mm = load3(a.scratch + (size_t)(n - n_a) * 3 * L, L, (int)gid);
         const StepFwd<T> f = step_forward(mm, e, w, dt, sig, al, ha, a.ftab[n + 1 - a.tab0],
                                           a.ftab[n + 2 - a.tab0]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) lam[c] = step_adjoint(f, e, dt, al, ha, lam[c], g[c]);
+        for (int c = 0; c < 3; ++c) lam[c] = step_adjoint(f, e, dt, al, ha, lam[c], gs[c]);
+        if ((n & 3) == 0) flush();
         if (n == piece_begin) {
+            flush();
             if (live) {
                 double *out = a.pieces + (size_t)(piece - a.piece_base) * kTransfer * P + p;
 #pragma unroll

@@ -67,9 +67,38 @@ std::vector<T> cast_vec(const std::vector<double> &v)
     return std::vector<T>(v.begin(), v.end());
 }
 
-template <typename T, int kShape, bool kArray, bool kRenorm>
-void launch_heun(const mp::IndepArgs<T> &a, int grid, hipStream_t s, const float *htab)
+// Whether FP32 Heun over n_waves integrating waves runs on the two-wave
+// kernel (a normal-generating wave beside each integrating wave): it pays while
+// the integrating waves are at most one per SIMD of the current device.
+// MAGPY_AMD_HEUN_TWO_WAVE=0/1 forces the choice.
+bool two_wave_pays(int n_waves)
 {
+    if (const char *env = std::getenv("MAGPY_AMD_HEUN_TWO_WAVE")) return std::atoi(env) != 0;
+    int dev = 0, cus = 0;
+    MP_HIP(hipGetDevice(&dev));
+    MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return n_waves <= 4 * cus;
+}
+
+// grid: blocks of kBlock lanes (n_waves = 4 grid integrating waves)
+template <typename T, int kShape, bool kArray, bool kRenorm>
+void launch_heun(const mp::IndepArgs<T> &a, int grid, hipStream_t s, const float *htab,
+                 bool two_wave)
+{
+    if constexpr (sizeof(T) == 4 && !kArray) {
+        if (two_wave) {
+            // one (integrating, producing) wave pair per workgroup; two or
+            // four pairs per workgroup measure the same or slower
+            const dim3 g2((unsigned)(grid * (mp::kBlock / mp::kWave))), b2(2 * mp::kWave);
+            if (htab)
+                hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, mp::kConstant, true, kRenorm>), g2,
+                                   b2, 0, s, a, htab);
+            else
+                hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, kShape, false, kRenorm>), g2, b2,
+                                   0, s, a, htab);
+            return;
+        }
+    }
     if constexpr (sizeof(T) == 4) {
         if (htab) {  // the table replaces per-lane field evaluation: one variant per noise kind
             hipLaunchKernelGGL((mp::heun_indep_kernel<T, mp::kConstant, kArray, true, kRenorm>),
@@ -84,7 +113,7 @@ void launch_heun(const mp::IndepArgs<T> &a, int grid, hipStream_t s, const float
 template <typename T, int kShape, bool kArray>
 void launch_step(const mp::IndepArgs<T> &a, int scheme, double eps, int max_iter,
                  unsigned long long *it, unsigned long long *fl, int grid, hipStream_t s,
-                 const float *htab)
+                 const float *htab, bool two_wave)
 {
     if constexpr (sizeof(T) == 8) {
         if (scheme == MP_SCHEME_IMPLICIT_MIDPOINT) {
@@ -95,21 +124,21 @@ void launch_step(const mp::IndepArgs<T> &a, int scheme, double eps, int max_iter
         }
     }
     if (a.renorm)
-        launch_heun<T, kShape, kArray, true>(a, grid, s, htab);
+        launch_heun<T, kShape, kArray, true>(a, grid, s, htab, two_wave);
     else
-        launch_heun<T, kShape, kArray, false>(a, grid, s, htab);
+        launch_heun<T, kShape, kArray, false>(a, grid, s, htab, two_wave);
 }
 
 template <typename T, bool kArray>
 void launch_shape(int shape_kind, const mp::IndepArgs<T> &a, int scheme, double eps,
                   int max_iter, unsigned long long *it, unsigned long long *fl, int grid,
-                  hipStream_t s, const float *htab)
+                  hipStream_t s, const float *htab, bool tw)
 {
     switch (shape_kind) {
-    case mp::kSine: launch_step<T, mp::kSine, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab); break;
-    case mp::kSquare: launch_step<T, mp::kSquare, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab); break;
-    case mp::kConstant: launch_step<T, mp::kConstant, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab); break;
-    default: launch_step<T, mp::kMixed, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab); break;
+    case mp::kSine: launch_step<T, mp::kSine, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab, tw); break;
+    case mp::kSquare: launch_step<T, mp::kSquare, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab, tw); break;
+    case mp::kConstant: launch_step<T, mp::kConstant, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab, tw); break;
+    default: launch_step<T, mp::kMixed, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab, tw); break;
     }
 }
 
@@ -202,6 +231,7 @@ struct GroupRun : GroupBase {
     int max_iter = 1000;
     double eps = 1e-9;
     bool array_noise = false, want_traj = false, want_ens = false, cluster = false;
+    bool two_wave = false;     // FP32 Heun on heun_indep_pc_kernel
     int64_t S = 0, s_next = 0, spl = 1;
     std::vector<int64_t> sched, slot_particle;
     DevBuf<T> d_ex, d_ey, d_ez, d_k, d_al, d_dt, d_sig, d_sr, d_happ, d_mx, d_my, d_mz, d_samples;
@@ -348,6 +378,8 @@ struct GroupRun : GroupBase {
         }
         grid = (n + block - 1) / block;
         n_waves = (grid * block) / mp::kWave;
+        two_wave = sizeof(T) == 4 && !cluster && !array_noise && scheme == MP_SCHEME_HEUN &&
+                   two_wave_pays(n_waves);
         if (want_traj) d_samples.alloc((size_t)S * 3 * (size_t)n);
         if (want_ens) {
             // each recorded sample writes all of its (component, wave) slots;
@@ -443,9 +475,11 @@ struct GroupRun : GroupBase {
             if (cluster)
                 launch_cluster(s, htab);
             else if (array_noise)
-                launch_shape<T, true>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s, htab);
+                launch_shape<T, true>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s,
+                                      htab, false);
             else
-                launch_shape<T, false>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s, htab);
+                launch_shape<T, false>(shape_kind, a, scheme, eps, max_iter, d_it.p, d_fl.p, grid, s,
+                                       htab, two_wave);
             MP_HIP(hipGetLastError());
             ++launches;
             n_done = n1;

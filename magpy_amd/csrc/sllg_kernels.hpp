@@ -97,17 +97,30 @@ struct Applied<double, kShape> {
 };
 
 // L(m, v) = -m x v + alpha (v |m|^2 - m (m.v))
+// a * b + c with one rounding.  The step arithmetic below is written with
+// explicit fused multiply-adds under `fp contract(off)`, so which products
+// fuse is fixed by the source and not by the surrounding kernel: every kernel
+// that evaluates a step (one- and two-wave Heun, the sensitivity forward pass
+// and its recompute) produces the same bits from the same inputs.
+template <typename T>
+__device__ inline T fmad(T a, T b, T c);
+template <>
+__device__ inline float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <>
+__device__ inline double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
 __device__ inline void llg_map(T mx, T my, T mz, T vx, T vy, T vz, T al, T &lx, T &ly, T &lz)
 {
-    const T cx = my * vz - mz * vy;
-    const T cy = mz * vx - mx * vz;
-    const T cz = mx * vy - my * vx;
-    const T d = mx * vx + my * vy + mz * vz;
-    const T q = mx * mx + my * my + mz * mz;
-    lx = al * (vx * q - mx * d) - cx;
-    ly = al * (vy * q - my * d) - cy;
-    lz = al * (vz * q - mz * d) - cz;
+#pragma clang fp contract(off)
+    const T cx = fmad(my, vz, -(mz * vy));
+    const T cy = fmad(mz, vx, -(mx * vz));
+    const T cz = fmad(mx, vy, -(my * vx));
+    const T d = fmad(mx, vx, fmad(my, vy, mz * vz));
+    const T q = fmad(mx, mx, fmad(my, my, mz * mz));
+    lx = fmad(al, fmad(vx, q, -(mx * d)), -cx);
+    ly = fmad(al, fmad(vy, q, -(my * d)), -cy);
+    lz = fmad(al, fmad(vz, q, -(mz * d)), -cz);
 }
 
 // ------------------------------------------------------------------ arguments
@@ -155,9 +168,16 @@ __device__ inline void wave_sum3(double &x, double &y, double &z)
     }
 }
 
+// wave of the calling lane in a grid of one-wave-per-64-particles blocks
+__device__ inline int grid_wave()
+{
+    return (int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+}
+
+// sample s of particle p; `wave` indexes the 64 particles of the calling wave
 template <typename T>
 __device__ inline void record_sample(const IndepArgs<T> &a, int64_t s, int p, bool live, T mx,
-                                     T my, T mz, double w)
+                                     T my, T mz, double w, int wave)
 {
     if (a.samples != nullptr && live) {
         T *base = a.samples + (size_t)s * 3 * (size_t)a.n;
@@ -171,7 +191,6 @@ __device__ inline void record_sample(const IndepArgs<T> &a, int64_t s, int p, bo
         double z = live ? (double)mz * w : 0.0;
         wave_sum3<T>(x, y, z);
         if ((threadIdx.x & (kWave - 1)) == 0) {
-            const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
             double *dst = a.ens_partial + (size_t)s * 3 * (size_t)a.n_waves;
             dst[wave] = x;
             dst[(size_t)a.n_waves + wave] = y;
@@ -191,23 +210,24 @@ template <typename T, bool kRenorm>
 __device__ inline void heun_update(T &mx, T &my, T &mz, T sx, T sy, T sz, T h1, T h2, T kex,
                                    T key, T kez, T edx, T edy, T edz, T dt, T al)
 {
-    const T sz1 = h1 * dt + sz, sz2 = h2 * dt + sz;
-    T d = mx * kex + my * key + mz * kez;
-    T vx = d * edx + sx, vy = d * edy + sy, vz = d * edz + sz1;
+#pragma clang fp contract(off)
+    const T sz1 = fmad(h1, dt, sz), sz2 = fmad(h2, dt, sz);
+    T d = fmad(mx, kex, fmad(my, key, mz * kez));
+    T vx = fmad(d, edx, sx), vy = fmad(d, edy, sy), vz = fmad(d, edz, sz1);
     T ax, ay, az;
     llg_map(mx, my, mz, vx, vy, vz, al, ax, ay, az);
     const T tx = mx + ax, ty = my + ay, tz = mz + az;
-    d = tx * kex + ty * key + tz * kez;
-    vx = d * edx + sx;
-    vy = d * edy + sy;
-    vz = d * edz + sz2;
+    d = fmad(tx, kex, fmad(ty, key, tz * kez));
+    vx = fmad(d, edx, sx);
+    vy = fmad(d, edy, sy);
+    vz = fmad(d, edz, sz2);
     T bx, by, bz;
     llg_map(tx, ty, tz, vx, vy, vz, al, bx, by, bz);
-    mx += T(0.5) * (ax + bx);
-    my += T(0.5) * (ay + by);
-    mz += T(0.5) * (az + bz);
+    mx = fmad(T(0.5), ax + bx, mx);
+    my = fmad(T(0.5), ay + by, my);
+    mz = fmad(T(0.5), az + bz, mz);
     if constexpr (kRenorm) {  // lib/simulation.cpp:379-387
-        const T nrm = sqrt(mx * mx + my * my + mz * mz);
+        const T nrm = sqrt(fmad(mx, mx, fmad(my, my, mz * mz)));
         mx = mx / nrm;
         my = my / nrm;
         mz = mz / nrm;
@@ -326,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a,
                                         edz, dt, al);
                 if (n + 1 == next_rec) {
                     do {
-                        record_sample(a, s, p, live, mx, my, mz, msw);
+                        record_sample(a, s, p, live, mx, my, mz, msw, grid_wave());
                         ++s;
                         next_rec = s < a.S ? a.sched[s] : INT64_MAX;
                     } while (next_rec == n + 1);
@@ -342,6 +362,138 @@ __global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a,
             for (int j = 0; j < 5; ++j) hc[j] = hn[j];
         }
     }
+    if (live) {
+        a.mx[p] = mx;
+        a.my[p] = my;
+        a.mz[p] = mz;
+        if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[p] |= 1u;
+    }
+}
+
+// ------------------------------------------------------------------ Heun, two waves
+// The FP32 step block for independent particles with the work of 64
+// particles split over a workgroup of two waves: wave 1 (producer) generates
+// the scaled Wiener normals of 4-step groups and, without a shared field
+// table, the applied field at both stage times of each step, into an LDS ring
+// of two halves of kGroups groups; wave 0 (integrator) runs the Heun steps
+// from the other half; the two swap halves at a barrier.  The integrator's
+// dependency chain then carries only the Heun arithmetic, and the producer's
+// independent work fills the SIMD cycles the chain leaves idle, which pays
+// where the particles are too few to give each SIMD several integrating
+// waves.  Every value is computed by the same expression as in
+// heun_indep_kernel, so the results are the same bit for bit.
+template <bool kTable>
+struct HeunRing {
+    static constexpr int kGroups = kTable ? 4 : 2;
+    static constexpr int kRows = kTable ? 12 : 20;   // normals, then (h1, h2) of each step
+};
+
+template <typename T, int kShape, bool kTable, bool kRenorm>
+__global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a,
+                                                                  const float *__restrict__ htab)
+{
+    static_assert(sizeof(T) == 4, "the two-wave kernel is FP32");
+    constexpr int K = HeunRing<kTable>::kGroups, R = HeunRing<kTable>::kRows;
+    __shared__ T ring[2][K][R][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool producer = __builtin_amdgcn_readfirstlane(threadIdx.x) >= kWave;
+    const int wave = blockIdx.x;
+    const int p = wave * kWave + lane;
+    const bool live = p < a.n;
+    const int q = live ? p : a.n - 1;
+    const int64_t n0 = a.n0, n1 = a.n1;
+    const int64_t g_begin = n0 >> 2, g_end = (n1 + 3) >> 2;
+    const int64_t n_chunks = (g_end - g_begin + K - 1) / K;
+
+    // producer inputs
+    const T sig = a.sig[q], amp = a.happ[q];
+    const double fdt = a.fdt[q];
+    const int shape = a.shape[q];
+    PhiloxKeys keys;
+    keys.init(a.key0[q], a.key1[q]);
+    const uint32_t pid = a.pid[q];
+    // integrator state
+    T mx = a.mx[q], my = a.my[q], mz = a.mz[q];
+    const T ex = a.ex[q], ey = a.ey[q], ez = a.ez[q];
+    const T k = a.k[q], al = a.alpha[q], dt = a.dt[q];
+    const T kex = k * ex, key = k * ey, kez = k * ez, edx = ex * dt, edy = ey * dt, edz = ez * dt;
+    const double msw = a.msw[q];
+    int64_t s = a.s_next;
+    int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+    // the integrator takes issue priority over co-resident waves
+    if (!producer) __builtin_amdgcn_s_setprio(3);
+    // pass c: the producer fills chunk c, the integrator runs chunk c - 1
+    for (int64_t c = 0; c <= n_chunks; ++c) {
+        if (producer) {
+            if (c < n_chunks) {
+                T (*half)[R][kWave] = ring[c & 1];
+                for (int kk = 0; kk < K; ++kk) {
+                    const int64_t g = g_begin + c * K + kk;
+                    if (g >= g_end) break;
+                    T z[12];
+                    group_normals(g, pid, a.tag, keys, sig, z);
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) half[kk][i][lane] = z[i];
+                    if constexpr (!kTable) {
+                        const double base = (double)(4 * g);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const double c1 = (base + (double)(u + 1)) * fdt;
+                            half[kk][12 + 2 * u][lane] = Applied<T, kShape>::at(shape, amp, c1);
+                            half[kk][13 + 2 * u][lane] = Applied<T, kShape>::at(shape, amp, c1 + fdt);
+                        }
+                    }
+                }
+            }
+        } else if (c > 0) {
+            T (*half)[R][kWave] = ring[(c - 1) & 1];
+            for (int kk = 0; kk < K; ++kk) {
+                const int64_t g = g_begin + (c - 1) * K + kk;
+                if (g >= g_end) break;
+                T z[12], h1[4], h2[4];
+#pragma unroll
+                for (int i = 0; i < 12; ++i) z[i] = half[kk][i][lane];
+                if constexpr (kTable) {
+                    const float *tg = htab + 4 * (g - g_begin);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        h1[u] = tg[u];
+                        h2[u] = tg[u + 1];
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        h1[u] = half[kk][12 + 2 * u][lane];
+                        h2[u] = half[kk][13 + 2 * u][lane];
+                    }
+                }
+                const bool fast = 4 * g >= n0 && 4 * g + 4 <= n1 && next_rec > 4 * g + 4;
+                if (fast) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        heun_update<T, kRenorm>(mx, my, mz, z[3 * u], z[3 * u + 1], z[3 * u + 2],
+                                                h1[u], h2[u], kex, key, kez, edx, edy, edz, dt, al);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t n = 4 * g + u;
+                        if (n < n0 || n >= n1) continue;
+                        heun_update<T, kRenorm>(mx, my, mz, z[3 * u], z[3 * u + 1], z[3 * u + 2],
+                                                h1[u], h2[u], kex, key, kez, edx, edy, edz, dt, al);
+                        if (n + 1 == next_rec) {
+                            do {
+                                record_sample(a, s, p, live, mx, my, mz, msw, wave);
+                                ++s;
+                                next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+                            } while (next_rec == n + 1);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (producer) return;
     if (live) {
         a.mx[p] = mx;
         a.my[p] = my;
@@ -519,7 +671,7 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
             }
             if (n + 1 == next_rec) {
                 do {
-                    record_sample(a, s, p, live, mx, my, mz, msw);
+                    record_sample(a, s, p, live, mx, my, mz, msw, grid_wave());
                     ++s;
                     next_rec = s < a.S ? a.sched[s] : INT64_MAX;
                 } while (next_rec == n + 1);
@@ -762,7 +914,7 @@ __global__ __launch_bounds__(kBlock) void implicit_indep_kernel(ImplicitArgs ia)
         }
         if (n + 1 == next_rec) {
             do {
-                record_sample(a, s, p, live, x0[0], x0[1], x0[2], msw);
+                record_sample(a, s, p, live, x0[0], x0[1], x0[2], msw, grid_wave());
                 ++s;
                 next_rec = s < a.S ? a.sched[s] : INT64_MAX;
             } while (next_rec == n + 1);
