@@ -60,7 +60,7 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (published)
 # device copy; see profiles/heun_indep_pc_fp32_65536_ac_field.md.
 TRAFFIC_BYTES_DEFAULT_LAUNCH = 13.30e6
 DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256),
-            "replicas": (4096, 64), "adjoint": (16384, 1 << 20)}
+            "replicas": (4096, 64), "adjoint": (16384, 1048000)}
 
 
 def ac_field_inputs(n, seed=2024):
@@ -268,11 +268,13 @@ def adjoint_bench(args):
     """Gradient throughput of the differentiation workload: 16,384 particles
     of the AC-field distribution, theta = (K, alpha, H, median r) at
     (23 kJ/m^3, 0.1, 20 mT, 10 nm), checkpoints every 1,000 steps.  A bench
-    step is one block of time steps through the recording forward pass (the
-    forward kernel plus the adjoint maps of the block's segments); the run is
-    warmup + steps blocks long and ends with the reverse pass (a scan over the
-    sampling intervals, timed and added).  value = particle-steps of the timed
-    blocks / (their wall time + the reverse pass)."""
+    step is about --block time steps through the recording forward pass (the
+    forward kernel plus the adjoint maps of the block's segments; the library
+    runs whole blocks of steps, so a step ends on the first block boundary at
+    or past its share); the run is warmup + steps such steps long and ends
+    with the reverse pass (a scan over the sampling intervals, timed and
+    added).  value = particle-steps of the timed steps / (their wall time +
+    the reverse pass)."""
     import magpy_amd
     core = magpy_amd.core
     P = args.particles or DEFAULTS["adjoint"][0]
@@ -288,14 +290,18 @@ def adjoint_bench(args):
     os.environ["MAGPY_AMD_ADJOINT_STEPS_PER_LAUNCH"] = str(B)
     run = core.Adjoint(shape, ax, ax, inp["seeds"], 400e3, 300.0, 3e5, 1e-14, N, ks, 1000, "fp32")
     run.begin(theta, True)
+    done0 = 0
     for b in range(args.warmup):
-        run.forward_advance(B)
+        done0 = run.forward_advance(B * (b + 1) - done0) if done0 < B * (b + 1) else done0
     info0 = run.info()   # waits for the warmup blocks' maps
     t0 = time.perf_counter()
+    done1 = done0
     for b in range(args.steps):
-        # returns when the block's forward kernel is done; its maps overlap
-        # the next block's forward kernel
-        run.forward_advance(B)
+        # returns when the forward kernels are done; the last block's maps
+        # overlap the next call's forward kernels
+        goal = B * (args.warmup + b + 1)
+        if done1 < goal:
+            done1 = run.forward_advance(goal - done1)
     info1 = run.info()   # waits for the last block's maps
     t_fwd = time.perf_counter() - t0
     ens, fin, status = run.forward_result()
@@ -304,7 +310,7 @@ def adjoint_bench(args):
     grad = run.reverse(w, 1.0)
     t_rev = time.perf_counter() - t0
     info2 = run.info()
-    ps = float(P) * B * args.steps
+    ps = float(P) * (done1 - done0)
     value = ps / (t_fwd + t_rev)
     fwd_kernel_s = (info1["forward_ms"] - info0["forward_ms"]) / 1e3
     rec_kernel_s = (info1["reverse_ms"] - info0["reverse_ms"]) / 1e3
@@ -319,6 +325,7 @@ def adjoint_bench(args):
                                "every 1,000 steps; step = %d time steps of the recording "
                                "forward pass" % B,
                    "particles": P, "time_steps_per_step": B, "run_length_steps": N,
+                   "time_steps_timed": int(done1 - done0),
                    "samples": len(ks), "checkpoint_bytes": info2["checkpoint_bytes"],
                    "scratch_bytes": info2["scratch_bytes"], "forward_status": int(status),
                    "gradient_finite": bool(np.all(np.isfinite(grad)))},

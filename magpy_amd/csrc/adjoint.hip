@@ -282,6 +282,7 @@ struct AdjointRun final : AdjointBase {
             (void)hipEventDestroy(t.e0);
             (void)hipEventDestroy(t.e1);
         }
+        for (hipEvent_t e : spare) (void)hipEventDestroy(e);
         for (int b = 0; b < 2; ++b) {
             if (ev_fwd[b]) (void)hipEventDestroy(ev_fwd[b]);
             if (ev_tr[b]) (void)hipEventDestroy(ev_tr[b]);
@@ -344,13 +345,24 @@ struct AdjointRun final : AdjointBase {
         hipEvent_t e0, e1;
     };
     std::vector<Timing> timings;
+    std::vector<hipEvent_t> spare;   // finished timing events, reused
+
+    hipEvent_t timing_event()
+    {
+        hipEvent_t e = nullptr;
+        if (!spare.empty()) {
+            e = spare.back();
+            spare.pop_back();
+        } else {
+            MP_HIP(hipEventCreate(&e));
+        }
+        return e;
+    }
 
     template <typename F>
     void timed(double &acc, hipStream_t st, F &&f)
     {
-        Timing t{&acc, nullptr, nullptr};
-        MP_HIP(hipEventCreate(&t.e0));
-        MP_HIP(hipEventCreate(&t.e1));
+        Timing t{&acc, timing_event(), timing_event()};
         timings.push_back(t);
         MP_HIP(hipEventRecord(t.e0, st));
         f();
@@ -371,8 +383,8 @@ struct AdjointRun final : AdjointBase {
             float ms = 0.f;
             MP_HIP(hipEventElapsedTime(&ms, t.e0, t.e1));
             *t.acc += ms;
-            (void)hipEventDestroy(t.e0);
-            (void)hipEventDestroy(t.e1);
+            spare.push_back(t.e0);
+            spare.push_back(t.e1);
         }
         timings.resize(keep);
     }
