@@ -407,6 +407,8 @@ def grad_check(args):
     cases = ((3e5, "full"), (3e7, "compressed"))
     if args.grad_check_quick:
         cases = cases[1:]
+    elif args.grad_check_full:
+        cases = cases[:1]
     for f, tag in cases:
         end = 2.0 / f
         k, n_steps = core.sample_schedule(1e-14, end, S)
@@ -563,6 +565,8 @@ def main():
     ap.add_argument("--sar-resume", default="", help="npz checkpoint to resume --sar-run from")
     ap.add_argument("--grad-check", action="store_true",
                     help="run the full differentiation workload and a finite-difference check")
+    ap.add_argument("--grad-check-full", action="store_true",
+                    help="with --grad-check: only the full 300 kHz gradient run")
     ap.add_argument("--grad-check-quick", action="store_true",
                     help="with --grad-check: only the finite-difference check")
     ap.add_argument("--noise-tag", type=int, default=0,
