@@ -53,6 +53,13 @@ sys.path.insert(0, ROOT)
 MU0 = 1.25663706e-6
 METRIC = "particle-steps/s (sLLG Heun, 1 GPU) at SAR err ≤0.5% vs FP64; plain vs batched"
 FLOP_PER_PARTICLE_STEP = 278          # Heun, N = 1, no renorm
+# implicit midpoint, N = 1 (lib/integrators.cpp:576-651 with Newton,
+# lib/optimisation.cpp:81-149): per Newton iteration, and per step besides
+# the iterations (Euler guess with its sde and Jacobians, tolerance, final
+# update); counted on the oracle's arithmetic by oracle/flop_count.py under
+# the convention that gives the Heun step its 278 (tests/test_flop_count.py)
+FLOP_IMPLICIT_PER_NEWTON = 503
+FLOP_IMPLICIT_PER_STEP = 365
 FLOP_PER_PAIR_STEP = 48               # 24 FLOP per ordered pair, two field evaluations
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (published)
 # HBM bytes of one default launch (65,536 particles x 131,072 steps), from
@@ -652,10 +659,15 @@ def main():
             "flop_per_time_step": flop_per_step}
     else:
         fin = ses.finish()
-        line["roofline"] = {"bound": "valu", "achieved": None, "peak": PEAK_TFLOPS[prec],
-                            "unit": "TFLOP/s", "frac": None, "traffic": None,
-                            "avg_launch_ms": avg_launch_s * 1e3, "launches": launches}
-        line["newton_iters_per_step"] = fin["newton_iters"] / (n_particles * max(fin["n_steps"], 1))
+        iters = fin["newton_iters"] / (n_particles * max(fin["n_steps"], 1))
+        flop_step = FLOP_IMPLICIT_PER_STEP + iters * FLOP_IMPLICIT_PER_NEWTON
+        achieved = flop_step * n_particles * steps_per_launch / avg_launch_s / 1e12
+        line["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[prec],
+                            "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[prec],
+                            "traffic": None, "avg_launch_ms": avg_launch_s * 1e3,
+                            "launches": launches, "flop_per_particle_step": flop_step,
+                            "flop_per_newton_iteration": FLOP_IMPLICIT_PER_NEWTON}
+        line["newton_iters_per_step"] = iters
         line["newton_failures"] = fin["newton_failures"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload, inp)
