@@ -350,6 +350,37 @@ def adjoint_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def library_gemm(args):
+    """The library's GEMM (torch.matmul -> hipBLASLt) at the replica
+    product's shape, 4096 x 1536 x 1536: one BF16 pass and FP32, HIP-event
+    timing over 50 calls on torch's stream."""
+    import torch
+    M, N, K = 4096, 1536, 1536
+    g = torch.Generator(device="cuda").manual_seed(5)
+    a = torch.randn(M, K, device="cuda", generator=g)
+    bt = torch.randn(N, K, device="cuda", generator=g).t().contiguous()
+    ah, bth = a.bfloat16(), bt.bfloat16()
+
+    def timeit(f, n=50):
+        for _ in range(5):
+            f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n * 1e3
+
+    out = {"library_gemm": "torch.matmul %dx%dx%d" % (M, N, K)}
+    for name, f in (("bf16", lambda: torch.matmul(ah, bth)), ("fp32", lambda: torch.matmul(a, bt))):
+        us = timeit(f)
+        out[name + "_us"] = us
+        out[name + "_tflops"] = 2.0 * M * N * K / us / 1e6
+    print(json.dumps(out), flush=True)
+
+
 def sar_and_weights(ens, time_s, theta_h, f):
     """SAR = -mu0 f trapz(H, <M_z>) over the last period (magpy/results.py:
     188-217) with H = theta_h sin(2 pi f t); returns (sar, dSAR/d ens,
@@ -572,6 +603,8 @@ def main():
     ap.add_argument("--sar-resume", default="", help="npz checkpoint to resume --sar-run from")
     ap.add_argument("--grad-check", action="store_true",
                     help="run the full differentiation workload and a finite-difference check")
+    ap.add_argument("--library-gemm", action="store_true",
+                    help="time the library GEMM at the replica product's shape")
     ap.add_argument("--grad-check-full", action="store_true",
                     help="with --grad-check: only the full 300 kHz gradient run")
     ap.add_argument("--grad-check-quick", action="store_true",
@@ -581,6 +614,9 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         copy_calibration(args.calibrate_copy)
+        return
+    if args.library_gemm:
+        library_gemm(args)
         return
     if args.grad_check:
         grad_check(args)
