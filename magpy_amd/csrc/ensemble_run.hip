@@ -67,13 +67,16 @@ std::vector<T> cast_vec(const std::vector<double> &v)
     return std::vector<T>(v.begin(), v.end());
 }
 
-// Whether FP32 Heun over n_waves integrating waves runs on the two-wave
-// kernel (a normal-generating wave beside each integrating wave): it pays while
-// the integrating waves are at most one per SIMD of the current device.
-// MAGPY_AMD_HEUN_TWO_WAVE=0/1 forces the choice.
-bool two_wave_pays(int n_waves)
+// Whether Heun over n_waves integrating waves runs on the two-wave
+// kernel (a normal-generating wave beside each integrating wave): in FP32 it
+// pays while the integrating waves are at most one per SIMD of the current
+// device; in FP64 the one-wave kernel is faster (65,536 particles: 5.7e10
+// against 4.3e10 particle-steps/s).  MAGPY_AMD_HEUN_TWO_WAVE=0/1 forces the
+// choice in either precision.
+bool two_wave_pays(int n_waves, bool fp32)
 {
     if (const char *env = std::getenv("MAGPY_AMD_HEUN_TWO_WAVE")) return std::atoi(env) != 0;
+    if (!fp32) return false;
     int dev = 0, cus = 0;
     MP_HIP(hipGetDevice(&dev));
     MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -85,17 +88,20 @@ template <typename T, int kShape, bool kArray, bool kRenorm>
 void launch_heun(const mp::IndepArgs<T> &a, int grid, hipStream_t s, const float *htab,
                  bool two_wave)
 {
-    if constexpr (sizeof(T) == 4 && !kArray) {
+    if constexpr (!kArray) {
         if (two_wave) {
             // one (integrating, producing) wave pair per workgroup; two or
             // four pairs per workgroup measure the same or slower
             const dim3 g2((unsigned)(grid * (mp::kBlock / mp::kWave))), b2(2 * mp::kWave);
-            if (htab)
-                hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, mp::kConstant, true, kRenorm>), g2,
-                                   b2, 0, s, a, htab);
-            else
-                hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, kShape, false, kRenorm>), g2, b2,
-                                   0, s, a, htab);
+            if constexpr (sizeof(T) == 4) {
+                if (htab) {
+                    hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, mp::kConstant, true, kRenorm>),
+                                       g2, b2, 0, s, a, htab);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, kShape, false, kRenorm>), g2, b2, 0, s,
+                               a, htab);
             return;
         }
     }
@@ -231,7 +237,7 @@ struct GroupRun : GroupBase {
     int max_iter = 1000;
     double eps = 1e-9;
     bool array_noise = false, want_traj = false, want_ens = false, cluster = false;
-    bool two_wave = false;     // FP32 Heun on heun_indep_pc_kernel
+    bool two_wave = false;     // Heun on heun_indep_pc_kernel
     int64_t S = 0, s_next = 0, spl = 1;
     std::vector<int64_t> sched, slot_particle;
     DevBuf<T> d_ex, d_ey, d_ez, d_k, d_al, d_dt, d_sig, d_sr, d_happ, d_mx, d_my, d_mz, d_samples;
@@ -378,8 +384,8 @@ struct GroupRun : GroupBase {
         }
         grid = (n + block - 1) / block;
         n_waves = (grid * block) / mp::kWave;
-        two_wave = sizeof(T) == 4 && !cluster && !array_noise && scheme == MP_SCHEME_HEUN &&
-                   two_wave_pays(n_waves);
+        two_wave = !cluster && !array_noise && scheme == MP_SCHEME_HEUN &&
+                   two_wave_pays(n_waves, sizeof(T) == 4);
         if (want_traj) d_samples.alloc((size_t)S * 3 * (size_t)n);
         if (want_ens) {
             // each recorded sample writes all of its (component, wave) slots;

@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a,
 }
 
 // ------------------------------------------------------------------ Heun, two waves
-// The FP32 step block for independent particles with the work of 64
+// The step block for independent particles with the work of 64
 // particles split over a workgroup of two waves: wave 1 (producer) generates
 // the scaled Wiener normals of 4-step groups and, without a shared field
 // table, the applied field at both stage times of each step, into an LDS ring
@@ -382,9 +382,11 @@ __global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a,
 // where the particles are too few to give each SIMD several integrating
 // waves.  Every value is computed by the same expression as in
 // heun_indep_kernel, so the results are the same bit for bit.
-template <bool kTable>
+// Ring halves of 4 groups (24 KB, FP32 with a table), 2 groups (20 KB, FP32
+// with per-lane fields) or 1 group (20 KB, FP64): four workgroups of a CU fit.
+template <typename T, bool kTable>
 struct HeunRing {
-    static constexpr int kGroups = kTable ? 4 : 2;
+    static constexpr int kGroups = sizeof(T) == 8 ? 1 : kTable ? 4 : 2;
     static constexpr int kRows = kTable ? 12 : 20;   // normals, then (h1, h2) of each step
 };
 
@@ -392,8 +394,8 @@ template <typename T, int kShape, bool kTable, bool kRenorm>
 __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a,
                                                                   const float *__restrict__ htab)
 {
-    static_assert(sizeof(T) == 4, "the two-wave kernel is FP32");
-    constexpr int K = HeunRing<kTable>::kGroups, R = HeunRing<kTable>::kRows;
+    static_assert(sizeof(T) == 4 || !kTable, "the shared field table is FP32");
+    constexpr int K = HeunRing<T, kTable>::kGroups, R = HeunRing<T, kTable>::kRows;
     __shared__ T ring[2][K][R][kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const bool producer = __builtin_amdgcn_readfirstlane(threadIdx.x) >= kWave;
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
 
     // producer inputs
     const T sig = a.sig[q], amp = a.happ[q];
-    const double fdt = a.fdt[q];
+    const double fdt = a.fdt[q], fred = a.fred[q], dtd = a.dtd[q];
     const int shape = a.shape[q];
     PhiloxKeys keys;
     keys.init(a.key0[q], a.key1[q]);
@@ -438,9 +440,18 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
                         const double base = (double)(4 * g);
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
-                            const double c1 = (base + (double)(u + 1)) * fdt;
-                            half[kk][12 + 2 * u][lane] = Applied<T, kShape>::at(shape, amp, c1);
-                            half[kk][13 + 2 * u][lane] = Applied<T, kShape>::at(shape, amp, c1 + fdt);
+                            if constexpr (sizeof(T) == 4) {
+                                const double c1 = (base + (double)(u + 1)) * fdt;
+                                half[kk][12 + 2 * u][lane] = Applied<T, kShape>::at(shape, amp, c1);
+                                half[kk][13 + 2 * u][lane] =
+                                    Applied<T, kShape>::at(shape, amp, c1 + fdt);
+                            } else {
+                                const double t1 = (base + (double)(u + 1)) * dtd;
+                                half[kk][12 + 2 * u][lane] =
+                                    Applied<T, kShape>::at(shape, amp, fred, t1);
+                                half[kk][13 + 2 * u][lane] =
+                                    Applied<T, kShape>::at(shape, amp, fred, t1 + dtd);
+                            }
                         }
                     }
                 }

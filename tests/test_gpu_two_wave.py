@@ -1,4 +1,4 @@
-"""The two-wave FP32 Heun kernel equals the one-wave kernel bit for bit.
+"""The two-wave Heun kernel equals the one-wave kernel bit for bit.
 
 For ensembles of at most one integrating wave per SIMD the library runs
 heun_indep_pc_kernel (one wave generating the normals and field values into
@@ -6,9 +6,10 @@ LDS, one integrating); larger ensembles run heun_indep_kernel.  Both compute
 every value with the same expressions, so forcing either kernel
 (MAGPY_AMD_HEUN_TWO_WAVE) on the same ensemble must give identical samples,
 final states and ensemble sums: with a shared field table (every member on
-one field), with per-lane fields (members on different frequencies and
-shapes), with samples inside 4-step groups, launch blocks that do not end on
-a group, a particle count that is not a multiple of 64, and renormalisation.
+one field, FP32), with per-lane fields (members on different frequencies
+and shapes), with samples inside 4-step groups, launch blocks that do not end
+on a group, a particle count that is not a multiple of 64, renormalisation,
+in FP32 and FP64.
 """
 import numpy as np
 import pytest
@@ -36,25 +37,27 @@ def ensemble(P, seed, mixed):
     return e
 
 
-def run_both(monkeypatch, **kw):
+def run_both(monkeypatch, precision, **kw):
     out = {}
     for tw in ("0", "1"):
         monkeypatch.setenv("MAGPY_AMD_HEUN_TWO_WAVE", tw)
-        out[tw] = core.simulate_ensemble(precision="fp32", **kw)
+        out[tw] = core.simulate_ensemble(precision=precision, **kw)
     return out["0"], out["1"]
 
 
-@pytest.mark.parametrize("P,mixed,renorm,block", [(4096, False, False, 0), (1000, True, False, 0),
-                                                  (777, False, True, 1001), (2048, True, True, 998)])
-def test_two_wave_equals_one_wave(monkeypatch, P, mixed, renorm, block):
-    print("CASE two_wave", P, mixed, renorm, block, flush=True)
+@pytest.mark.parametrize("precision,P,mixed,renorm,block", [
+    ("fp32", 4096, False, False, 0), ("fp32", 1000, True, False, 0),
+    ("fp32", 777, False, True, 1001), ("fp32", 2048, True, True, 998),
+    ("fp64", 4096, False, False, 0), ("fp64", 777, True, True, 1001)])
+def test_two_wave_equals_one_wave(monkeypatch, precision, P, mixed, renorm, block):
+    print("CASE two_wave", precision, P, mixed, renorm, block, flush=True)
     if block:
         monkeypatch.setenv("MAGPY_AMD_STEPS_PER_LAUNCH", str(block))
     e = ensemble(P, 17 + P, mixed)
     dt = 1e-14
     # 4,003 steps; 37 samples fall at steps that are not multiples of 4
-    one, two = run_both(monkeypatch, time_step=dt, end_time=4002.5 * dt, max_samples=37,
-                        renorm=renorm, **e)
+    one, two = run_both(monkeypatch, precision, time_step=dt, end_time=4002.5 * dt,
+                        max_samples=37, renorm=renorm, **e)
     assert one["n_steps"] == two["n_steps"]
     assert np.all(one["status"] == 0) and np.all(two["status"] == 0)
     for c in "xyz":
