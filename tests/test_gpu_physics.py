@@ -139,3 +139,32 @@ def test_deterministic_order_two():
     slope = np.polyfit(np.log(ms), np.log(err), 1)[0]
     print("deterministic errors", dict(zip(ms, err)), "order", slope, flush=True)
     assert 1.8 < slope < 2.3
+
+
+def test_config1_plain_fp64_ensemble():
+    """The reference's CPU-runnable case: one 12 nm particle (K = 23 kJ/m^3,
+    Ms = 400 kA/m, alpha = 0.1, 300 K, axis and m0 along +z, no field), 1,000
+    realisations with their own seeds, 1e5 Heun steps of 1e-14 s in FP64.
+    Three realisations equal the oracle run on the same Philox stream over
+    the whole time series; the 1,000 final m_z are tested against the
+    Boltzmann distribution on [0, 1] (Kolmogorov-Smirnov)."""
+    print("CASE config1_plain_fp64", flush=True)
+    from scipy.stats import kstest
+    from oracle import oracle as O
+    M, S = 1000, 101
+    e = ensemble(M, [0, 0, 1.0], 0.0, 300.0, seed=9)
+    dt, end = 1e-14, 1e-9
+    o = core.simulate_ensemble(time_step=dt, end_time=end, max_samples=S, precision="fp64",
+                               trajectories=True, **e)
+    assert np.all(o["status"] == 0)
+    for m in (0, 499, 999):
+        r = O.full_dynamics([R], [K], [[0, 0, 1.0]], [[0, 0, 1.0]], [[0, 0, 0]], MS, AL, 300.0,
+                            False, False, False, 1e-9, dt, end, S, int(e["seeds"][m]),
+                            rng="philox")
+        for c in "xyz":
+            assert np.allclose(o[c][m], r[c][0], rtol=0, atol=MS * 1e-10), (m, c)
+    mz = last_sample_mz(o)
+    p = kstest(mz, boltzmann_cdf).pvalue
+    print("config 1: mean 1-mz", np.mean(1 - mz), "exact", boltzmann_mean_gap(), "KS p", p,
+          flush=True)
+    assert p > 1e-3
