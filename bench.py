@@ -350,6 +350,63 @@ def adjoint_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def replica_accuracy(args):
+    """Shared-geometry replicas (one 512-particle cluster, 15% volume
+    fraction, 20 mT / 300 kHz, 300 K, 1e5 Heun steps of 1e-14 s) with the
+    dipolar field as the BF16x3 product, the FP32 product, FP32 pair sums,
+    and FP64 pair sums (the reference), all on the same Philox stream:
+    deviation of the ensemble <M_z>(t) (101 samples, relative to the total
+    Ms) and rms deviation of the final reduced states from the FP64 run, the partial-cycle energy -mu0 int H d<M_z> of each run
+    (1 ns is 3e-4 of a field period, so this is not yet a loop area), and
+    the dipolar-field error of each product on the FP64 run's final states.
+    --particles sets the replica count (default 256)."""
+    import magpy_amd
+    core = magpy_amd.core
+    R = args.particles or 256
+    inp = replica_inputs(R)
+    n = 512
+    kw = dict(time_step=1e-14, end_time=1e5 * 1e-14, max_samples=101, interactions=True,
+              trajectories=False, ensemble_sum=True)
+    print("device:", core.device_info(0), flush=True)
+    runs = {}
+    for name, prec, dip in (("fp64_pairs", "fp64", "pairs"), ("fp32_pairs", "fp32", "pairs"),
+                            ("gemm_f32", "fp32", "gemm_f32"),
+                            ("gemm_bf16x3", "fp32", "gemm_bf16x3")):
+        t0 = time.perf_counter()
+        o = core.simulate_ensemble(precision=prec, dipolar=dip, **kw, **inp)
+        runs[name] = o
+        print(json.dumps({"run": name, "wall_s": time.perf_counter() - t0,
+                          "status": int(np.max(o["status"]))}), flush=True)
+    ref = runs["fp64_pairs"]
+    t = ref["time"]
+    h = ref["field"]
+    ms_total = 400e3 * n * R
+
+    def energy(o):
+        mz = o["ensemble_m"][:, 2] / R         # <M_z> per replica, A/m x particles
+        return float(-MU0 * np.trapezoid(h, mz))
+
+    final64 = ref["final_state"].reshape(R, n, 3)     # reduced states
+    h64 = core.dipolar_field(inp["location"][:n], inp["radius"][:n], inp["anisotropy"][:n],
+                             400e3, final64, mode="pairs")
+    scale = np.linalg.norm(h64, axis=(1, 2))
+    line = {"replica_accuracy": "%d replicas x 512 particles, 1e5 steps" % R,
+            "energy_fp64_J_per_m3_x_particles": energy(ref)}
+    for name in ("fp32_pairs", "gemm_f32", "gemm_bf16x3"):
+        o = runs[name]
+        d_ens = np.max(np.abs(o["ensemble_m"][:, 2] - ref["ensemble_m"][:, 2])) / ms_total
+        d_fin = np.sqrt(np.mean(np.sum((o["final_state"] - ref["final_state"]) ** 2, axis=1)))
+        entry = {"ens_mz_max_dev_rel": float(d_ens), "final_state_rms_dev": float(d_fin),
+                 "energy": energy(o)}
+        if name.startswith("gemm"):
+            hp = core.dipolar_field(inp["location"][:n], inp["radius"][:n], inp["anisotropy"][:n],
+                                    400e3, final64, mode=name)
+            entry["field_err_max_rel"] = float(np.max(np.linalg.norm(hp - h64, axis=(1, 2)) / scale))
+        line[name] = entry
+    line["time_s"] = [float(t[0]), float(t[-1])]
+    print(json.dumps(line), flush=True)
+
+
 def library_gemm(args):
     """The library's GEMM (torch.matmul -> hipBLASLt) at the replica
     product's shape, 4096 x 1536 x 1536: one BF16 pass and FP32, HIP-event
@@ -603,6 +660,8 @@ def main():
     ap.add_argument("--sar-resume", default="", help="npz checkpoint to resume --sar-run from")
     ap.add_argument("--grad-check", action="store_true",
                     help="run the full differentiation workload and a finite-difference check")
+    ap.add_argument("--replica-accuracy", action="store_true",
+                    help="replica runs with each dipolar product against the FP64 pair sums")
     ap.add_argument("--library-gemm", action="store_true",
                     help="time the library GEMM at the replica product's shape")
     ap.add_argument("--grad-check-full", action="store_true",
@@ -617,6 +676,9 @@ def main():
         return
     if args.library_gemm:
         library_gemm(args)
+        return
+    if args.replica_accuracy:
+        replica_accuracy(args)
         return
     if args.grad_check:
         grad_check(args)
