@@ -730,7 +730,8 @@ def main():
                    "particles": n_particles, "members": len(inp["member_offsets"]) - 1,
                    "time_steps_per_step": block, "run_length_steps": info1["n_steps"],
                    "precision": prec, "state_finite": bool(np.all(np.isfinite(st))),
-                   "cluster_block": info1["cluster_block"], "dipolar": info1["dipolar"],
+                   "cluster_block": info1["cluster_block"],
+                   "cluster_lanes": info1["cluster_lanes"], "dipolar": info1["dipolar"],
                    "step_index": int(step)},
     }
     if flop_per_step is not None and args.workload == "replicas":

@@ -161,10 +161,12 @@ int mp_session_create(const mp_ensemble_spec *spec, int want_trajectories, int w
  * *steps_done (optional) = steps taken so far after the call */
 int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done);
 /* total steps of the run, kernel launches / device milliseconds so far, the
- * block size of interacting clusters (0 without pair sums) and the
- * MP_DIPOLAR_* path the run uses (0 without interacting clusters) */
+ * block size (lanes per workgroup) of interacting clusters (0 without pair
+ * sums), the MP_DIPOLAR_* path the run uses (0 without interacting clusters)
+ * and the lanes per particle of the pair-sum kernel (1 or 2; 0 without) */
 int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                    double *kernel_ms, int32_t *cluster_block, int32_t *dipolar);
+                    double *kernel_ms, int32_t *cluster_block, int32_t *dipolar,
+                    int32_t *cluster_lanes);
 /* reduced state [P][3] and the step index it belongs to */
 int mp_session_state(mp_session *session, double *state, int64_t *step);
 /* run to the end and fill `out` as mp_ensemble_run would */

@@ -68,7 +68,8 @@ cdef extern from "magpy_amd.h" nogil:
                           mp_session **session)
     int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done)
     int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                        double *kernel_ms, int32_t *cluster_block, int32_t *dipolar)
+                        double *kernel_ms, int32_t *cluster_block, int32_t *dipolar,
+                        int32_t *cluster_lanes)
     int mp_dipolar_field(const double *location, const double *radius, const double *anisotropy,
                          size_t n_particles, double magnetisation, const double *m, size_t reps,
                          int mode, double *h, int device)
@@ -488,9 +489,10 @@ cdef class Session:
     def info(self):
         cdef int64_t tot = 0, launches = 0
         cdef double ms = 0
-        cdef int32_t cblock = 0, dip = 0
-        _check(mp_session_info(self.s, &tot, &launches, &ms, &cblock, &dip))
+        cdef int32_t cblock = 0, dip = 0, clanes = 0
+        _check(mp_session_info(self.s, &tot, &launches, &ms, &cblock, &dip, &clanes))
         return {'n_steps': tot, 'launches': launches, 'kernel_ms': ms, 'cluster_block': cblock,
+                'cluster_lanes': clanes,
                 'dipolar': DIPOLAR_NAMES.get(dip, 'none') if dip else 'none'}
 
     def state(self):

@@ -238,6 +238,7 @@ struct GroupRun : GroupBase {
     double eps = 1e-9;
     bool array_noise = false, want_traj = false, want_ens = false, cluster = false;
     bool two_wave = false;     // Heun on heun_indep_pc_kernel
+    int lanes = 1;             // lanes per cluster slot (1 or 2)
     int64_t S = 0, s_next = 0, spl = 1;
     std::vector<int64_t> sched, slot_particle;
     DevBuf<T> d_ex, d_ey, d_ez, d_k, d_al, d_dt, d_sig, d_sr, d_happ, d_mx, d_my, d_mz, d_samples;
@@ -254,9 +255,10 @@ struct GroupRun : GroupBase {
     mp::IndepArgs<T> a{};
 
     // cluster_block == 0: one slot per particle of the group (independent
-    // particles); otherwise members packed into blocks of cluster_block lanes.
+    // particles); otherwise members packed into blocks of cluster_block lanes,
+    // cluster_lanes (1 or 2) lanes per slot.
     GroupRun(const simulation::EnsembleSpec &sp, const Plan &plan, int gi, bool traj, bool ens,
-             int cluster_block)
+             int cluster_block, int cluster_lanes = 1)
     {
         const int64_t b = plan.group_begin[(size_t)gi], e = plan.group_begin[(size_t)gi + 1];
         const int64_t P = sp.member_offsets[sp.n_members];
@@ -282,7 +284,8 @@ struct GroupRun : GroupBase {
                 i += len;
             }
             block = cluster_block;
-            pack_clusters(members, block, slot_particle, cbase, csize);
+            lanes = cluster_lanes;
+            pack_clusters(members, block / lanes, slot_particle, cbase, csize);
         } else {
             slot_particle.assign(plan.perm.begin() + b, plan.perm.begin() + e);
         }
@@ -382,7 +385,8 @@ struct GroupRun : GroupBase {
             d_cbase.upload(cbase);
             d_csize.upload(csize);
         }
-        grid = (n + block - 1) / block;
+        // cluster slots come in whole blocks of block / lanes
+        grid = cluster ? n / (block / lanes) : (n + block - 1) / block;
         n_waves = (grid * block) / mp::kWave;
         two_wave = !cluster && !array_noise && scheme == MP_SCHEME_HEUN &&
                    two_wave_pays(n_waves, sizeof(T) == 4);
@@ -427,33 +431,40 @@ struct GroupRun : GroupBase {
 
     void launch_cluster(hipStream_t s, const float *htab)
     {
+        if (lanes == 2) launch_cluster_lanes<2>(s, htab);
+        else launch_cluster_lanes<1>(s, htab);
+    }
+
+    template <int L>
+    void launch_cluster_lanes(hipStream_t s, const float *htab)
+    {
         mp::ClusterArgs<T> ca{a, d_px.p, d_py.p, d_pz.p, d_w.p, d_cbase.p, d_csize.p};
-        const size_t shmem = 3 * (size_t)block * sizeof(mp::Vec4<T>);
+        const size_t shmem = 3 * (size_t)(block / L) * sizeof(mp::Vec4<T>);
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, s, ca, htab);
         };
         const bool rn = a.renorm != 0;
         if (htab) {
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true>);
-            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true, L>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false, L>);
             return;
         }
         switch (shape_kind) {
         case mp::kSine:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true>);
-            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true, L>);
+            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false, L>);
             break;
         case mp::kSquare:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true>);
-            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true, L>);
+            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false, L>);
             break;
         case mp::kConstant:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true>);
-            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true, L>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false, L>);
             break;
         default:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true>);
-            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true, L>);
+            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false, L>);
             break;
         }
     }
@@ -826,6 +837,7 @@ struct mp_session {
     bool want_traj = false, want_ens = false;
     int device = 0;
     int cluster_block = 0;
+    int cluster_lanes = 0;
     int dipolar = 0;
     double kernel_ms = 0.0;
 };
@@ -949,54 +961,103 @@ void validate(const EnsembleSpec &sp)
 
 namespace {
 
-// Block size for interacting clusters: the smallest power of two of at
-// least one wave that holds the largest cluster.
-int cluster_block_size(int64_t nmax)
+// Layout of the interacting-cluster kernel: `block` lanes per workgroup and
+// `lanes` (1 or 2) lanes per particle slot, so a workgroup holds
+// block / lanes slots, which must take the largest cluster.
+struct ClusterLayout {
+    int block = 0, lanes = 1;
+};
+
+// The smallest power-of-two block of at least one wave whose slots hold the
+// largest cluster (0: none up to 1024 lanes).
+int cluster_min_block(int64_t nmax, int lanes)
 {
-    if (const char *env = std::getenv("MAGPY_AMD_CLUSTER_BLOCK")) {
-        const int v = std::atoi(env);
-        if (v >= nmax && v % mp::kWave == 0 && v <= 1024) return v;
-    }
     for (int b = mp::kWave; b <= 1024; b *= 2)
-        if (b >= nmax) return b;
+        if (b / lanes >= nmax) return b;
     return 0;
 }
 
-// Autotuner of the cluster block size: for runs long enough to repay it,
-// time a few trial steps of the first schedule group with every candidate
-// block (powers of two from the smallest that holds the largest cluster up
-// to 1024 lanes) on throw-away copies of the state, and keep the fastest.
-// Results are cached per (largest cluster, slots, precision, steps-per-group
-// bucket) for the life of the process.
-int autotune_cluster_block(const simulation::EnsembleSpec &sp, const Plan &plan, int64_t nmax,
-                           hipStream_t s)
+// Two lanes per particle pay while one lane per particle would leave fewer
+// than this many waves per CU: the pair sums of a slot run in half the time
+// and the extra waves hide the LDS latency of the pair loop.
+constexpr int kSplitBelowWavesPerCU = 16;
+
+// Lanes per slot forced by MAGPY_AMD_CLUSTER_LANES (1 or 2), else 0.
+int env_cluster_lanes()
 {
-    static std::map<std::tuple<int64_t, int64_t, int>, int> cache;
-    const auto key = std::make_tuple(nmax, (int64_t)plan.perm.size(), (int)sp.precision);
+    const char *env = std::getenv("MAGPY_AMD_CLUSTER_LANES");
+    if (env == nullptr) return 0;
+    return std::atoi(env) == 2 ? 2 : 1;
+}
+
+// Untimed choice of the layout: MAGPY_AMD_CLUSTER_LANES and
+// MAGPY_AMD_CLUSTER_BLOCK (lanes per workgroup) when set and valid, else two
+// lanes per slot for ensembles of few waves, and the smallest block.
+ClusterLayout cluster_layout(int64_t nmax, int64_t particles)
+{
+    ClusterLayout l;
+    if (const int forced = env_cluster_lanes()) {
+        l.lanes = forced;
+    } else {
+        int dev = 0, cus = 0;
+        MP_HIP(hipGetDevice(&dev));
+        MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        l.lanes = particles < (int64_t)kSplitBelowWavesPerCU * mp::kWave * cus ? 2 : 1;
+    }
+    if (cluster_min_block(nmax, l.lanes) == 0) l.lanes = 1;
+    if (const char *env = std::getenv("MAGPY_AMD_CLUSTER_BLOCK")) {
+        const int v = std::atoi(env);
+        if (v / l.lanes >= nmax && v % mp::kWave == 0 && v <= 1024) {
+            l.block = v;
+            return l;
+        }
+    }
+    l.block = cluster_min_block(nmax, l.lanes);
+    return l;
+}
+
+// Autotuner of the cluster layout: for runs long enough to repay it, time a
+// few trial steps of the first schedule group with every candidate (one and
+// two lanes per slot, unless MAGPY_AMD_CLUSTER_LANES fixes it, each with the
+// power-of-two blocks from the smallest that holds the largest cluster up to
+// 1024 lanes) on throw-away copies of the state, and keep the fastest.
+// Results are cached per (largest cluster, particles, precision) for the
+// life of the process.
+ClusterLayout autotune_cluster_layout(const simulation::EnsembleSpec &sp, const Plan &plan,
+                                      int64_t nmax, hipStream_t s)
+{
+    static std::map<std::tuple<int64_t, int64_t, int, int>, ClusterLayout> cache;
+    const int forced = env_cluster_lanes();
+    const auto key = std::make_tuple(nmax, (int64_t)plan.perm.size(), (int)sp.precision, forced);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     const int64_t trial_steps = 64;
     hipEvent_t e0, e1;
     MP_HIP(hipEventCreate(&e0));
     MP_HIP(hipEventCreate(&e1));
-    int best = 0;
+    ClusterLayout best;
     float best_ms = 0.f;
-    for (int b = cluster_block_size(nmax); b != 0 && b <= 1024; b *= 2) {
-        std::unique_ptr<GroupBase> g;
-        if (sp.precision == MP_PRECISION_FP32)
-            g.reset(new GroupRun<float>(sp, plan, 0, false, false, b));
-        else
-            g.reset(new GroupRun<double>(sp, plan, 0, false, false, b));
-        g->enqueue(4, s);  // warm the code object
-        MP_HIP(hipEventRecord(e0, s));
-        g->enqueue(4 + trial_steps, s);
-        MP_HIP(hipEventRecord(e1, s));
-        MP_HIP(hipStreamSynchronize(s));
-        float ms = 0.f;
-        MP_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (best == 0 || ms < best_ms) {
-            best = b;
-            best_ms = ms;
+    for (int lanes = 1; lanes <= 2; ++lanes) {
+        if (forced != 0 && lanes != forced) continue;
+        const int b0 = cluster_min_block(nmax, lanes);
+        for (int b = b0; b != 0 && b <= 1024; b *= 2) {
+            std::unique_ptr<GroupBase> g;
+            if (sp.precision == MP_PRECISION_FP32)
+                g.reset(new GroupRun<float>(sp, plan, 0, false, false, b, lanes));
+            else
+                g.reset(new GroupRun<double>(sp, plan, 0, false, false, b, lanes));
+            g->enqueue(4, s);  // warm the code object
+            MP_HIP(hipEventRecord(e0, s));
+            g->enqueue(4 + trial_steps, s);
+            MP_HIP(hipEventRecord(e1, s));
+            MP_HIP(hipStreamSynchronize(s));
+            float ms = 0.f;
+            MP_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (best.block == 0 || ms < best_ms) {
+                best.block = b;
+                best.lanes = lanes;
+                best_ms = ms;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -1071,18 +1132,19 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
                     "Exceeded allocated random number array size (noise_steps < steps)");
     // interacting clusters: one block holds whole clusters, so the block must
     // be at least the largest cluster
-    int cluster_block = 0;
+    ClusterLayout layout;
     int64_t nmax = 1;
     if (sp.interactions) {
         for (int64_t m = 0; m < sp.n_members; ++m)
             nmax = std::max(nmax, sp.member_offsets[m + 1] - sp.member_offsets[m]);
         if (nmax > 1) {
-            cluster_block = cluster_block_size(nmax);
-            if (cluster_block == 0)
+            layout = cluster_layout(nmax, (int64_t)plan.perm.size());
+            if (layout.block == 0)
                 throw simulation::Error(MP_ERR_UNSUPPORTED,
                                         "clusters of more than 1024 particles are not supported");
         }
     }
+    int cluster_block = layout.block;
     // replicas of one cluster: the dipolar field as one product per stage
     int gemm = 0;
     if (cluster_block != 0 && G == 1 && sp.scheme == MP_SCHEME_HEUN &&
@@ -1107,14 +1169,18 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
     // pair sums: pick the block size by timing the candidates on long runs
     if (cluster_block != 0 && std::getenv("MAGPY_AMD_CLUSTER_BLOCK") == nullptr &&
         sp.noise == MP_NOISE_PHILOX && plan.group_steps[0] >= 20000)
-        cluster_block = autotune_cluster_block(sp, plan, nmax, ss->sg.s);
+        layout = autotune_cluster_layout(sp, plan, nmax, ss->sg.s);
+    cluster_block = layout.block;
     ss->dipolar = cluster_block ? MP_DIPOLAR_PAIRS : 0;
     ss->cluster_block = cluster_block;
+    ss->cluster_lanes = cluster_block ? layout.lanes : 0;
     for (int g = 0; g < G; ++g) {
         if (sp.precision == MP_PRECISION_FP32 && sp.scheme == MP_SCHEME_HEUN)
-            ss->groups.emplace_back(new GroupRun<float>(sp, plan, g, traj, ens, cluster_block));
+            ss->groups.emplace_back(
+                new GroupRun<float>(sp, plan, g, traj, ens, cluster_block, layout.lanes));
         else
-            ss->groups.emplace_back(new GroupRun<double>(sp, plan, g, traj, ens, cluster_block));
+            ss->groups.emplace_back(
+                new GroupRun<double>(sp, plan, g, traj, ens, cluster_block, layout.lanes));
     }
     return ss.release();
 }
@@ -1316,7 +1382,8 @@ int mp_session_advance(mp_session *session, int64_t n_steps, int64_t *steps_done
 }
 
 int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launches,
-                    double *kernel_ms, int32_t *cluster_block, int32_t *dipolar)
+                    double *kernel_ms, int32_t *cluster_block, int32_t *dipolar,
+                    int32_t *cluster_lanes)
 {
     return guarded([&] {
         if (!session) throw simulation::Error(MP_ERR_ARG, "session is required");
@@ -1330,6 +1397,7 @@ int mp_session_info(mp_session *session, int64_t *n_steps_total, int64_t *launch
         if (kernel_ms) *kernel_ms = session->kernel_ms;
         if (cluster_block) *cluster_block = session->cluster_block;
         if (dipolar) *dipolar = session->dipolar;
+        if (cluster_lanes) *cluster_lanes = session->cluster_lanes;
     });
 }
 

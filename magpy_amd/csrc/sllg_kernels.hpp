@@ -551,58 +551,101 @@ struct alignas(16) Vec4 {
     T x, y, z, w;
 };
 
-// Dipolar field at lane `self` from lanes [b, e) of its cluster except itself,
-// summed in increasing j (the reference's order, lib/field.cpp:193-202).  One
-// loop with the self term masked: split loops j < i, j > i would give the
-// lanes of a wave complementary trip counts and run both in full.
+// One dipolar pair term: the field at pi of the moment mj (weight pj.w) at
+// pj, added to (ax, ay, az); `skip` masks the particle's own lane.
 template <typename T>
-__device__ inline void dipolar_sum(const Vec4<T> *pos, const Vec4<T> *m, int self, int b, int e,
-                                   T &hx, T &hy, T &hz)
+__device__ inline void dipolar_term(const Vec4<T> &pi, const Vec4<T> &pj, const Vec4<T> &mj,
+                                    bool skip, T &ax, T &ay, T &az)
 {
-    const Vec4<T> pi = pos[self];
-    T ax = T(0), ay = T(0), az = T(0);
-    auto term = [&](const Vec4<T> &pj, const Vec4<T> &mj, bool skip) {
-        const T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        const T r2 = skip ? T(1) : dx * dx + dy * dy + dz * dz;
-        T rinv;
-        if constexpr (sizeof(T) == 4) rinv = __builtin_amdgcn_rsqf(r2);
-        else rinv = 1.0 / sqrt(r2);
-        const T rinv2 = rinv * rinv;
-        const T c = skip ? T(0) : pj.w * rinv2 * rinv;
-        const T t = T(3) * (mj.x * dx + mj.y * dy + mj.z * dz) * rinv2;
-        ax += c * (t * dx - mj.x);
-        ay += c * (t * dy - mj.y);
-        az += c * (t * dz - mj.z);
-    };
-    int j = b;
-    for (; j + 4 <= e; j += 4) {
-        Vec4<T> pj[4], mj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            pj[u] = pos[j + u];
-            mj[u] = m[j + u];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) term(pj[u], mj[u], j + u == self);
-    }
-    for (; j < e; ++j) term(pos[j], m[j], j == self);
-    hx = ax;
-    hy = ay;
-    hz = az;
+#pragma clang fp contract(off)
+    const T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+    const T r2 = skip ? T(1) : fmad(dx, dx, fmad(dy, dy, dz * dz));
+    T rinv;
+    if constexpr (sizeof(T) == 4) rinv = __builtin_amdgcn_rsqf(r2);
+    else rinv = 1.0 / sqrt(r2);
+    const T rinv2 = rinv * rinv;
+    const T c = skip ? T(0) : pj.w * rinv2 * rinv;
+    const T t = T(3) * fmad(mj.x, dx, fmad(mj.y, dy, mj.z * dz)) * rinv2;
+    ax = fmad(c, fmad(t, dx, -mj.x), ax);
+    ay = fmad(c, fmad(t, dy, -mj.y), ay);
+    az = fmad(c, fmad(t, dz, -mj.z), az);
 }
 
-template <typename T, int kShape, bool kTable, bool kRenorm>
+// Dipolar field at lane `self` from lanes [b, e) of its cluster except itself
+// (lib/field.cpp:193-202), as two sums taken in increasing j: over the lanes
+// at even offsets from b and over those at odd offsets, added at the end.
+// With kLanes = 1 one lane carries both sums in one loop; with kLanes = 2 the
+// two lanes of a particle (lanes 2i, 2i+1 of the wave) take one sum each
+// (`half`) and exchange them.  Either way the result is (even sum) + (odd
+// sum), the same bits.  The self term is masked rather than split out, so
+// every lane of a cluster runs the same trip count.
+template <typename T, int kLanes>
+__device__ inline void dipolar_field(const Vec4<T> *pos, const Vec4<T> *m, int self, int b,
+                                     int e, int half, T &hx, T &hy, T &hz)
+{
+    const Vec4<T> pi = pos[self];
+    if constexpr (kLanes == 1) {
+        T x0 = T(0), y0 = T(0), z0 = T(0), x1 = T(0), y1 = T(0), z1 = T(0);
+        int j = b;
+        for (; j + 4 <= e; j += 4) {
+            Vec4<T> pj[4], mj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                pj[u] = pos[j + u];
+                mj[u] = m[j + u];
+            }
+            dipolar_term(pi, pj[0], mj[0], j == self, x0, y0, z0);
+            dipolar_term(pi, pj[1], mj[1], j + 1 == self, x1, y1, z1);
+            dipolar_term(pi, pj[2], mj[2], j + 2 == self, x0, y0, z0);
+            dipolar_term(pi, pj[3], mj[3], j + 3 == self, x1, y1, z1);
+        }
+        // j - b is a multiple of 4: the rest start at an even offset
+        if (j < e) dipolar_term(pi, pos[j], m[j], j == self, x0, y0, z0);
+        if (j + 1 < e) dipolar_term(pi, pos[j + 1], m[j + 1], j + 1 == self, x1, y1, z1);
+        if (j + 2 < e) dipolar_term(pi, pos[j + 2], m[j + 2], j + 2 == self, x0, y0, z0);
+        hx = x0 + x1;
+        hy = y0 + y1;
+        hz = z0 + z1;
+    } else {
+        static_assert(kLanes == 2, "one or two lanes per particle");
+        T x = T(0), y = T(0), z = T(0);
+        int j = b + half;
+        for (; j + 6 < e; j += 8) {
+            Vec4<T> pj[4], mj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                pj[u] = pos[j + 2 * u];
+                mj[u] = m[j + 2 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) dipolar_term(pi, pj[u], mj[u], j + 2 * u == self, x, y, z);
+        }
+        for (; j < e; j += 2) dipolar_term(pi, pos[j], m[j], j == self, x, y, z);
+        hx = x + __shfl_xor(x, 1);
+        hy = y + __shfl_xor(y, 1);
+        hz = z + __shfl_xor(z, 1);
+    }
+}
+
+// kLanes lanes per slot: the block holds blockDim.x / kLanes slots; with two,
+// the lanes of a slot split its pair sums and repeat the rest of the step, and
+// the even lane alone writes the shared state, samples and results.
+template <typename T, int kShape, bool kTable, bool kRenorm, int kLanes>
 __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
                                                            const float *__restrict__ htab)
 {
+#pragma clang fp contract(off)
     const IndepArgs<T> &a = ca.base;
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int slots = blockDim.x / kLanes;
     Vec4<T> *pos = reinterpret_cast<Vec4<T> *>(smem_raw);
-    Vec4<T> *mb0 = pos + blockDim.x;
-    Vec4<T> *mb1 = mb0 + blockDim.x;
+    Vec4<T> *mb0 = pos + slots;
+    Vec4<T> *mb1 = mb0 + slots;
 
-    const int tid = threadIdx.x;
-    const int p = blockIdx.x * blockDim.x + tid;   // slot: the grid covers all slots exactly
+    const int tid = threadIdx.x / kLanes;           // block-relative slot
+    const int half = threadIdx.x % kLanes;
+    const bool lead = half == 0;
+    const int p = blockIdx.x * slots + tid;   // slot: the grid covers all slots exactly
     const int cn = ca.csize[p];
     const bool live = cn > 0;
     const int cb = ca.cbase[p], ce = cb + cn;
@@ -619,7 +662,7 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
     const uint32_t pid = a.pid[p];
     const int64_t n0 = a.n0, n1 = a.n1;
     const int64_t tab0 = 4 * (n0 >> 2);
-    pos[tid] = Vec4<T>{ca.px[p], ca.py[p], ca.pz[p], ca.w[p]};
+    if (lead) pos[tid] = Vec4<T>{ca.px[p], ca.py[p], ca.pz[p], ca.w[p]};
 
     int64_t s = a.s_next;
     int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
@@ -649,32 +692,32 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
                 h2 = Applied<T, kShape>::at(shape, amp, fred, t1 + dtd);
             }
             // stage 1
-            mb0[tid] = Vec4<T>{mx, my, mz, T(0)};
+            if (lead) mb0[tid] = Vec4<T>{mx, my, mz, T(0)};
             __syncthreads();
             T hx, hy, hz;
-            dipolar_sum(pos, mb0, tid, cb, ce, hx, hy, hz);
-            T d = mx * kex + my * key + mz * kez;
-            T vx = d * edx + (hx * dt + sx), vy = d * edy + (hy * dt + sy),
-              vz = d * edz + ((hz + h1) * dt + sz);
+            dipolar_field<T, kLanes>(pos, mb0, tid, cb, ce, half, hx, hy, hz);
+            T d = fmad(mx, kex, fmad(my, key, mz * kez));
+            T vx = fmad(d, edx, fmad(hx, dt, sx)), vy = fmad(d, edy, fmad(hy, dt, sy)),
+              vz = fmad(d, edz, fmad(hz + h1, dt, sz));
             T ax, ay, az;
             llg_map(mx, my, mz, vx, vy, vz, al, ax, ay, az);
             const T tx = mx + ax, ty = my + ay, tz = mz + az;
             // stage 2
-            mb1[tid] = Vec4<T>{tx, ty, tz, T(0)};
+            if (lead) mb1[tid] = Vec4<T>{tx, ty, tz, T(0)};
             __syncthreads();
-            dipolar_sum(pos, mb1, tid, cb, ce, hx, hy, hz);
-            d = tx * kex + ty * key + tz * kez;
-            vx = d * edx + (hx * dt + sx);
-            vy = d * edy + (hy * dt + sy);
-            vz = d * edz + ((hz + h2) * dt + sz);
+            dipolar_field<T, kLanes>(pos, mb1, tid, cb, ce, half, hx, hy, hz);
+            d = fmad(tx, kex, fmad(ty, key, tz * kez));
+            vx = fmad(d, edx, fmad(hx, dt, sx));
+            vy = fmad(d, edy, fmad(hy, dt, sy));
+            vz = fmad(d, edz, fmad(hz + h2, dt, sz));
             T bx, by, bz;
             llg_map(tx, ty, tz, vx, vy, vz, al, bx, by, bz);
-            mx += T(0.5) * (ax + bx);
-            my += T(0.5) * (ay + by);
-            mz += T(0.5) * (az + bz);
+            mx = fmad(T(0.5), ax + bx, mx);
+            my = fmad(T(0.5), ay + by, my);
+            mz = fmad(T(0.5), az + bz, mz);
             if constexpr (kRenorm) {
                 if (live) {
-                    const T nrm = sqrt(mx * mx + my * my + mz * mz);
+                    const T nrm = sqrt(fmad(mx, mx, fmad(my, my, mz * mz)));
                     mx = mx / nrm;
                     my = my / nrm;
                     mz = mz / nrm;
@@ -682,7 +725,7 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
             }
             if (n + 1 == next_rec) {
                 do {
-                    record_sample(a, s, p, live, mx, my, mz, msw, grid_wave());
+                    record_sample(a, s, p, live && lead, mx, my, mz, msw, grid_wave());
                     ++s;
                     next_rec = s < a.S ? a.sched[s] : INT64_MAX;
                 } while (next_rec == n + 1);
@@ -691,6 +734,7 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
 #pragma unroll
         for (int i = 0; i < 12; ++i) zc[i] = zn[i];
     }
+    if (!lead) return;
     a.mx[p] = mx;
     a.my[p] = my;
     a.mz[p] = mz;
