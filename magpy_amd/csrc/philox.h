@@ -45,6 +45,17 @@ struct PhiloxKeys {
     }
 };
 
+// a ^ b ^ c; on the device one v_bitop3_b32 (truth table 0x96) instead of
+// two v_xor_b32, which the compiler does not merge by itself.
+MP_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 MP_HD u32x4 philox4x32_10(u32x4 c, const PhiloxKeys &k)
 {
 #pragma unroll
@@ -52,9 +63,9 @@ MP_HD u32x4 philox4x32_10(u32x4 c, const PhiloxKeys &k)
         const uint64_t p0 = (uint64_t)kPhiloxM0 * c.x;
         const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
         u32x4 n;
-        n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k.k0[r];
+        n.x = xor3((uint32_t)(p1 >> 32), c.y, k.k0[r]);
         n.y = (uint32_t)p1;
-        n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k.k1[r];
+        n.z = xor3((uint32_t)(p0 >> 32), c.w, k.k1[r]);
         n.w = (uint32_t)p0;
         c = n;
     }
