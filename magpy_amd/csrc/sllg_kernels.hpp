@@ -38,11 +38,19 @@ template <>
 struct Num<float> {
     // Native transcendentals: v_log_f32 (log2), v_sqrt_f32, v_sin/v_cos_f32
     // (argument in revolutions, so sin(2 pi v) is one instruction).
+    // -2 ln 2: r^2 = -2 ln u = kRadius2 log2 u
+    static constexpr float kRadius2 = -1.38629436111989061883f;
     static __device__ inline void box_muller(uint32_t a, uint32_t b, float &z0, float &z1)
+    {
+        box_muller_r2(a, b, kRadius2, z0, z1);
+    }
+    // the pair scaled by s, given c = kRadius2 s^2
+    static __device__ inline void box_muller_r2(uint32_t a, uint32_t b, float c, float &z0,
+                                                float &z1)
     {
         const float u = fmaf((float)a, 0x1p-32f, 0x1p-33f);
         const float v = fmaf((float)b, 0x1p-32f, 0x1p-33f);
-        const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061883f * __builtin_amdgcn_logf(u));
+        const float r = __builtin_amdgcn_sqrtf(c * __builtin_amdgcn_logf(u));
         z0 = r * __builtin_amdgcn_cosf(v);
         z1 = r * __builtin_amdgcn_sinf(v);
     }
@@ -96,7 +104,6 @@ struct Applied<double, kShape> {
     }
 };
 
-// L(m, v) = -m x v + alpha (v |m|^2 - m (m.v))
 // a * b + c with one rounding.  The step arithmetic below is written with
 // explicit fused multiply-adds under `fp contract(off)`, so which products
 // fuse is fixed by the source and not by the surrounding kernel: every kernel
@@ -109,18 +116,24 @@ __device__ inline float fmad(float a, float b, float c) { return __builtin_fmaf(
 template <>
 __device__ inline double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// L(m, v) = -m x v + alpha (v |m|^2 - m (m.v)), the reference's drift and
+// diffusion applied to v (lib/llg.cpp:14-29, 92-106), evaluated as -w - alpha m x w
+// with w = m x v: m x (m x v) = m (m.v) - v |m|^2 for any m, so this is the
+// same map in 15 operations (two cross products and three fused
+// multiply-adds) instead of 21.
 template <typename T>
 __device__ inline void llg_map(T mx, T my, T mz, T vx, T vy, T vz, T al, T &lx, T &ly, T &lz)
 {
 #pragma clang fp contract(off)
-    const T cx = fmad(my, vz, -(mz * vy));
-    const T cy = fmad(mz, vx, -(mx * vz));
-    const T cz = fmad(mx, vy, -(my * vx));
-    const T d = fmad(mx, vx, fmad(my, vy, mz * vz));
-    const T q = fmad(mx, mx, fmad(my, my, mz * mz));
-    lx = fmad(al, fmad(vx, q, -(mx * d)), -cx);
-    ly = fmad(al, fmad(vy, q, -(my * d)), -cy);
-    lz = fmad(al, fmad(vz, q, -(mz * d)), -cz);
+    const T wx = fmad(my, vz, -(mz * vy));
+    const T wy = fmad(mz, vx, -(mx * vz));
+    const T wz = fmad(mx, vy, -(my * vx));
+    const T ux = fmad(my, wz, -(mz * wy));
+    const T uy = fmad(mz, wx, -(mx * wz));
+    const T uz = fmad(mx, wy, -(my * wx));
+    lx = fmad(-al, ux, -wx);
+    ly = fmad(-al, uy, -wy);
+    lz = fmad(-al, uz, -wz);
 }
 
 // ------------------------------------------------------------------ arguments
@@ -235,19 +248,36 @@ __device__ inline void heun_update(T &mx, T &my, T &mz, T sx, T sy, T sz, T h1, 
 }
 
 // Twelve Wiener normals of step group g (steps 4g..4g+3) scaled by sig.
+// FP32 folds sig into the Box-Muller radius, sig r = sqrt(-2 ln2 sig^2
+// log2 u), which saves two multiplies per pair; with sig = 1 the radius is
+// the unscaled one bit for bit.  (sig^2 stays a normal float for any sig
+// above 1e-19, far below the noise of any physical temperature.)
 template <typename T>
 __device__ inline void group_normals(int64_t g, uint32_t pid, uint32_t tag, const PhiloxKeys &keys,
                                      T sig, T z[12])
 {
+    if constexpr (sizeof(T) == 4) {
+        const float c = Num<float>::kRadius2 * (sig * sig);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const uint64_t c = (uint64_t)(3 * g + i);
-        const u32x4 w = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, tag}, keys);
-        Num<T>::box_muller(w.x, w.y, z[4 * i], z[4 * i + 1]);
-        Num<T>::box_muller(w.z, w.w, z[4 * i + 2], z[4 * i + 3]);
+        for (int i = 0; i < 3; ++i) {
+            const uint64_t n = (uint64_t)(3 * g + i);
+            const u32x4 w =
+                philox4x32_10(u32x4{(uint32_t)n, (uint32_t)(n >> 32), pid, tag}, keys);
+            Num<float>::box_muller_r2(w.x, w.y, c, z[4 * i], z[4 * i + 1]);
+            Num<float>::box_muller_r2(w.z, w.w, c, z[4 * i + 2], z[4 * i + 3]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const uint64_t n = (uint64_t)(3 * g + i);
+            const u32x4 w =
+                philox4x32_10(u32x4{(uint32_t)n, (uint32_t)(n >> 32), pid, tag}, keys);
+            Num<T>::box_muller(w.x, w.y, z[4 * i], z[4 * i + 1]);
+            Num<T>::box_muller(w.z, w.w, z[4 * i + 2], z[4 * i + 3]);
+        }
+#pragma unroll
+        for (int i = 0; i < 12; ++i) z[i] *= sig;
     }
-#pragma unroll
-    for (int i = 0; i < 12; ++i) z[i] *= sig;
 }
 
 // One block of steps [n0, n1) for independent particles.  Steps are walked in
@@ -391,8 +421,12 @@ struct HeunRing {
 };
 
 template <typename T, int kShape, bool kTable, bool kRenorm>
+// htab is deliberately not __restrict__: the integrator then reads it with
+// vector loads, which complete in order like its LDS reads, so the reads of
+// the next group can be in flight while a group is integrated (scalar loads
+// complete out of order and would make every wait a wait for all of them).
 __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a,
-                                                                  const float *__restrict__ htab)
+                                                                  const float *htab)
 {
     static_assert(sizeof(T) == 4 || !kTable, "the shared field table is FP32");
     constexpr int K = HeunRing<T, kTable>::kGroups, R = HeunRing<T, kTable>::kRows;
@@ -458,26 +492,47 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
             }
         } else if (c > 0) {
             T (*half)[R][kWave] = ring[(c - 1) & 1];
-            for (int kk = 0; kk < K; ++kk) {
-                const int64_t g = g_begin + (c - 1) * K + kk;
-                if (g >= g_end) break;
-                T z[12], h1[4], h2[4];
+            const int64_t gc = g_begin + (c - 1) * K;   // first group of the chunk (< g_end)
+            // group kk's normals and field values; loaded one group ahead so
+            // the LDS and scalar-load latency overlaps the previous group's steps
+            // (a group past the end of the block reads the last group's table
+            // entries and ring rows it does not use)
+            auto load_group = [&](int kk, T *zz, T *hh1, T *hh2) {
 #pragma unroll
-                for (int i = 0; i < 12; ++i) z[i] = half[kk][i][lane];
+                for (int i = 0; i < 12; ++i) zz[i] = half[kk][i][lane];
                 if constexpr (kTable) {
-                    const float *tg = htab + 4 * (g - g_begin);
+                    const int64_t gt = gc + kk < g_end ? gc + kk : g_end - 1;
+                    const float *tg = htab + 4 * (gt - g_begin);
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        h1[u] = tg[u];
-                        h2[u] = tg[u + 1];
+                        hh1[u] = tg[u];
+                        hh2[u] = tg[u + 1];
                     }
                 } else {
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
-                        h1[u] = half[kk][12 + 2 * u][lane];
-                        h2[u] = half[kk][13 + 2 * u][lane];
+                        hh1[u] = half[kk][12 + 2 * u][lane];
+                        hh2[u] = half[kk][13 + 2 * u][lane];
                     }
                 }
+            };
+            T zn[12], h1n[4], h2n[4];
+            load_group(0, zn, h1n, h2n);
+            // unrolled, so the prefetched registers need no copies (a copy of
+            // a scalar load's result would wait for it at once)
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+                const int64_t g = gc + kk;
+                if (g >= g_end) break;
+                T z[12], h1[4], h2[4];
+#pragma unroll
+                for (int i = 0; i < 12; ++i) z[i] = zn[i];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    h1[u] = h1n[u];
+                    h2[u] = h2n[u];
+                }
+                if (kk + 1 < K) load_group(kk + 1, zn, h1n, h2n);
                 const bool fast = 4 * g >= n0 && 4 * g + 4 <= n1 && next_rec > 4 * g + 4;
                 if (fast) {
 #pragma unroll
