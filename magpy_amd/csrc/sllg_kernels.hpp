@@ -420,11 +420,11 @@ struct HeunRing {
     static constexpr int kRows = kTable ? 12 : 20;   // normals, then (h1, h2) of each step
 };
 
-template <typename T, int kShape, bool kTable, bool kRenorm>
 // htab is deliberately not __restrict__: the integrator then reads it with
 // vector loads, which complete in order like its LDS reads, so the reads of
 // the next group can be in flight while a group is integrated (scalar loads
 // complete out of order and would make every wait a wait for all of them).
+template <typename T, int kShape, bool kTable, bool kRenorm>
 __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a,
                                                                   const float *htab)
 {
