@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "linalg3.h"
 #include "philox.h"
 
 namespace mp {
@@ -857,43 +858,6 @@ __device__ inline void diffusion_jac3(const double *m, double sr, double al, dou
     jb[18] = -al * sr * m[2];   jb[19] = sr;                 jb[20] = -al * sr * m[0];
     jb[21] = -sr;               jb[22] = -al * sr * m[2];    jb[23] = -al * sr * m[1];
     jb[24] = 2 * al * sr * m[0]; jb[25] = 2 * al * sr * m[2]; jb[26] = 0;
-}
-
-// LU with partial pivoting (dgetf2 order: reciprocal scaling) and solve of
-// J x = b in place.  Returns false on an exactly zero pivot.
-__device__ inline bool solve3(double *A, double *b)
-{
-    int piv[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        int p = j;
-        double best = fabs(A[j * 3 + j]);
-#pragma unroll
-        for (int i = j + 1; i < 3; ++i)
-            if (fabs(A[i * 3 + j]) > best) { best = fabs(A[i * 3 + j]); p = i; }
-        piv[j] = p;
-        if (A[p * 3 + j] == 0.0) return false;
-        if (p != j) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) { const double t = A[j * 3 + c]; A[j * 3 + c] = A[p * 3 + c]; A[p * 3 + c] = t; }
-        }
-        const double rcp = 1.0 / A[j * 3 + j];
-#pragma unroll
-        for (int i = j + 1; i < 3; ++i) A[i * 3 + j] *= rcp;
-#pragma unroll
-        for (int i = j + 1; i < 3; ++i)
-#pragma unroll
-            for (int c = j + 1; c < 3; ++c) A[i * 3 + c] -= A[i * 3 + j] * A[j * 3 + c];
-    }
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-        if (piv[j] != j) { const double t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
-    b[1] -= A[3] * b[0];
-    b[2] -= A[6] * b[0] + A[7] * b[1];
-    b[2] /= A[8];
-    b[1] = (b[1] - A[5] * b[2]) / A[4];
-    b[0] = (b[0] - A[1] * b[1] - A[2] * b[2]) / A[0];
-    return true;
 }
 
 template <int kShape, bool kArrayNoise>
