@@ -22,6 +22,11 @@ Other workloads (--workload):
             between sample steps) and the reverse scan; gradient of a
             weighted sum of <M_z> samples with respect to (K, alpha, H,
             median r).
+  dom       the discrete orientation model of the 65,536 AC-field particles
+            (two-state master equation, adaptive Cash-Karp RK45 in FP64,
+            tolerance 1e-12, 5 field cycles, 11 samples); one bench step is
+            one whole run, timed by HIP events around its kernel; value in
+            accepted RK45 particle-steps/s.
 
 --grad-check runs the full differentiation workload (2 cycles at 300 kHz,
 6.67e8 steps) and a finite-difference check of its gradients on a 30 MHz
@@ -61,13 +66,16 @@ FLOP_PER_PARTICLE_STEP = 278          # Heun, N = 1, no renorm
 FLOP_IMPLICIT_PER_NEWTON = 503
 FLOP_IMPLICIT_PER_STEP = 365
 FLOP_PER_PAIR_STEP = 48               # 24 FLOP per ordered pair, two field evaluations
+# one accepted Cash-Karp step of the master equation, counted on the oracle's
+# arithmetic (oracle/flop_count.py dom_flops())
+FLOP_DOM_PER_STEP = 401
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (published)
 # HBM bytes of one default launch (65,536 particles x 131,072 steps), from
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) scaled against a 1 GiB
 # device copy; see profiles/heun_indep_pc_fp32_65536_ac_field.md.
 TRAFFIC_BYTES_DEFAULT_LAUNCH = 9.27e6
 DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256),
-            "replicas": (4096, 64), "adjoint": (16384, 1048000)}
+            "replicas": (4096, 64), "adjoint": (16384, 1048000), "dom": (65536, 1)}
 
 
 def ac_field_inputs(n, seed=2024):
@@ -564,6 +572,113 @@ def grad_check(args):
         print(json.dumps(line), flush=True)
 
 
+DOM_RUN = dict(time_step=1e-12, end_time=5 / 3e5, max_samples=11, field_shape="sine",
+               field_amplitude=0.02 / MU0, field_frequency=3e5)
+
+
+def dom_inputs(n, seed=2024):
+    """the AC-field ensemble's particles (same radii), all in the up state"""
+    r = ac_field_inputs(n, seed)["radius"]
+    return dict(volume=4.0 / 3.0 * np.pi * r ** 3, anisotropy=np.full(n, 23e3),
+                temperature=np.full(n, 300.0), magnetisation=np.full(n, 400e3),
+                damping=np.full(n, 0.1), initial_probabilities=np.tile([1.0, 0.0], (n, 1)))
+
+
+def dom_cpu_baseline(inp, steps_per_particle, seconds=12.0):
+    """The oracle's orc_dom_ensemble_dynamics (simulation::dom_ensemble_dynamics
+    restated) on the host cores, particle by particle; accepted steps are the
+    device run's per-particle counts (the same adaptive algorithm)."""
+    from oracle import oracle as O
+    O.lib()
+    n_threads = min(os.cpu_count() or 1, 16)
+    P = len(inp["volume"])
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0}
+    t_stop = [0.0]
+
+    def particle(i):
+        hk = 2 * inp["anisotropy"][i] / MU0 / inp["magnetisation"][i]
+        O.dom_ensemble_dynamics(inp["volume"][i], inp["anisotropy"][i], inp["temperature"][i],
+                                inp["magnetisation"][i], inp["damping"][i], "sine",
+                                DOM_RUN["field_amplitude"] / hk, DOM_RUN["field_frequency"],
+                                inp["initial_probabilities"][i], DOM_RUN["time_step"],
+                                DOM_RUN["end_time"], DOM_RUN["max_samples"])
+        return int(steps_per_particle[i])
+
+    def worker():
+        while True:
+            with lock:
+                if time.perf_counter() > t_stop[0]:
+                    return
+                i = state["next"] % P
+                state["next"] += 1
+            ps = particle(i)
+            with lock:
+                state["done"] += ps
+
+    def run(nt, secs):
+        state["next"], state["done"] = 0, 0
+        t0 = time.perf_counter()
+        t_stop[0] = t0 + secs
+        th = [threading.Thread(target=worker) for _ in range(nt)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return state["done"] / (time.perf_counter() - t0), state["next"]
+
+    one, m1 = run(1, seconds / 4)
+    allc, ma = run(n_threads, seconds)
+    return {"value": allc, "unit": "particle-RK45-steps/s", "cores": n_threads, "kind": "port",
+            "value_1core": one,
+            "sample": "%d particles' whole runs of the FP64 oracle's dom_ensemble_dynamics "
+                      "(function-pointer master equation, reference rk45) on %d threads; "
+                      "1-core rate from %d particles" % (ma, n_threads, m1)}
+
+
+def dom_bench(args):
+    import magpy_amd
+    core = magpy_amd.core
+    n = args.particles or DEFAULTS["dom"][0]
+    inp = dom_inputs(n)
+    print("device:", core.device_info(0), flush=True)
+    for _ in range(args.warmup):
+        core.simulate_dom_ensemble(**inp, **DOM_RUN)
+    steps, kernel_s, wall, bad = 0, 0.0, 0.0, 0
+    per = None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        r = core.simulate_dom_ensemble(**inp, **DOM_RUN)
+        wall += time.perf_counter() - t0
+        steps += int(r["steps"].sum())
+        kernel_s += r["kernel_ms"] / 1e3
+        bad += int(np.count_nonzero(r["status"]))
+        per = r["steps"]
+    value = steps / kernel_s
+    achieved = FLOP_DOM_PER_STEP * steps / kernel_s / 1e12
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle-RK45-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": kernel_s / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "discrete orientation model of the 65,536-particle AC-field "
+                               "ensemble (two-state master equation, Cash-Karp RK45 FP64, "
+                               "tolerance 1e-12, 5 cycles of 20 mT / 300 kHz); step = one whole "
+                               "run of all particles",
+                   "particles": n, "rk45_steps_per_particle": float(per.mean()),
+                   "particles_flagged": bad},
+        "pcie_inclusive_value": steps / wall,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["fp64"],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"], "traffic": None,
+                     "avg_launch_ms": kernel_s / args.steps * 1e3,
+                     "flop_per_rk45_step": FLOP_DOM_PER_STEP},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = dom_cpu_baseline(inp, per)
+        line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
 def adjoint_cpu_baseline(inp, theta, seconds=12.0):
     """The oracle's FP64 forward-mode tangent (4 directions) on the host
     cores: gradient particle-steps/s on a bounded sample of the workload."""
@@ -694,6 +809,9 @@ def main():
 
     if args.workload == "adjoint":
         adjoint_bench(args)
+        return
+    if args.workload == "dom":
+        dom_bench(args)
         return
     import magpy_amd
     core = magpy_amd.core

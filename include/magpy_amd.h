@@ -305,9 +305,10 @@ typedef struct mp_dom_spec {
 } mp_dom_spec;
 /* time [S] (s), field [S] (A/m, particle 0's reduced field times its H_k, as
  * the reference returns it), mz [P][S] = p_up - p_down, status [P],
- * steps [P] accepted RK45 steps; any output may be NULL */
+ * steps [P] accepted RK45 steps, kernel_ms the device time of the
+ * integration kernel (HIP events around its launch); any output may be NULL */
 int mp_dom_run(const mp_dom_spec *spec, double *time, double *field, double *mz,
-               uint32_t *status, int64_t *steps);
+               uint32_t *status, int64_t *steps, double *kernel_ms);
 
 /* simulation::save_results (lib/simulation.cpp:38-63): writes prefix.mx,
  * prefix.my, prefix.mz, prefix.field and prefix.time, each the raw

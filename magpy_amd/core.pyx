@@ -131,7 +131,7 @@ cdef extern from "magpy_amd.h" nogil:
         int64_t max_steps
         int32_t device
     int mp_dom_run(const mp_dom_spec *spec, double *time, double *field, double *mz,
-                   uint32_t *status, int64_t *steps)
+                   uint32_t *status, int64_t *steps, double *kernel_ms)
     int mp_save_results(const char *prefix, const double *mx, const double *my,
                         const double *mz, const double *field, const double *time,
                         size_t n_samples)
@@ -747,7 +747,8 @@ def simulate_dom_ensemble(volume, anisotropy, temperature, magnetisation, dampin
                           int64_t field_n_components=1, int64_t max_steps=0, int device=0):
     """Discrete orientation model of every particle of an ensemble in one
     device launch (mp_dom_run).  Per-particle arrays (length P); returns
-    dict(time [S], field [S] (A/m), mz [P, S], status [P], steps [P])."""
+    dict(time [S], field [S] (A/m), mz [P, S], status [P], steps [P],
+    kernel_ms (device time of the integration kernel))."""
     v = _c(volume, np.float64).reshape(-1)
     P = v.shape[0]
     if P == 0:
@@ -788,10 +789,12 @@ def simulate_dom_ensemble(volume, anisotropy, temperature, magnetisation, dampin
     cdef uint32_t[::1] sv = status
     cdef int64_t[::1] stv = steps
     cdef int rc
+    cdef double kms = 0.0
     with nogil:
-        rc = mp_dom_run(&sp, &tv[0], &fv[0], &mzv[0, 0], &sv[0], &stv[0])
+        rc = mp_dom_run(&sp, &tv[0], &fv[0], &mzv[0, 0], &sv[0], &stv[0], &kms)
     _check(rc)
-    return {'time': time, 'field': field, 'mz': mz, 'status': status, 'steps': steps}
+    return {'time': time, 'field': field, 'mz': mz, 'status': status, 'steps': steps,
+            'kernel_ms': kms}
 
 
 cpdef simulate_dom(initial_probabilities, double volume, double anisotropy, double temperature,

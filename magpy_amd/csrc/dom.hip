@@ -34,7 +34,7 @@ double dom_field_host(int shape, double h, double f, int64_t n, double t)
 }
 
 void dom_run(const mp_dom_spec &sp, double *time, double *field, double *mz, uint32_t *status,
-             int64_t *steps)
+             int64_t *steps, double *kernel_ms)
 {
 #pragma clang fp contract(off)
     const int64_t P = sp.n_particles, S = sp.max_samples;
@@ -99,9 +99,19 @@ void dom_run(const mp_dom_spec &sp, double *time, double *field, double *mz, uin
     a.mz = dmz.p;
     a.status = dst.p;
     a.steps = dsteps.p;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MP_HIP(hipEventCreate(&e0));
+    MP_HIP(hipEventCreate(&e1));
+    MP_HIP(hipEventRecord(e0, nullptr));
     hipLaunchKernelGGL(mp::dom_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), 0, nullptr, a);
     MP_HIP(hipGetLastError());
+    MP_HIP(hipEventRecord(e1, nullptr));
     MP_HIP(hipDeviceSynchronize());
+    float ms = 0.f;
+    MP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
     std::vector<double> h((size_t)S * P);
     MP_HIP(hipMemcpy(h.data(), dmz.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     if (mz)
@@ -124,10 +134,10 @@ void dom_run(const mp_dom_spec &sp, double *time, double *field, double *mz, uin
 }  // namespace
 
 extern "C" int mp_dom_run(const mp_dom_spec *spec, double *time, double *field, double *mz,
-                          uint32_t *status, int64_t *steps)
+                          uint32_t *status, int64_t *steps, double *kernel_ms)
 {
     return guarded([&] {
         require(spec != nullptr, "spec is required");
-        dom_run(*spec, time, field, mz, status, steps);
+        dom_run(*spec, time, field, mz, status, steps, kernel_ms);
     });
 }

@@ -20,6 +20,17 @@ check.
                             (lib/optimisation.cpp:81-149) and the sde of
                             lib/simulation.cpp:306-325: FLOP per Newton
                             iteration and per step outside the iterations
+  dom_flops()               one accepted Cash-Karp step of integrator::rk45
+                            (lib/integrators.cpp:152-251) on the master
+                            equation of lib/dom.cpp:30-93 with the sine field
+                            of lib/field.cpp:37-40: FLOP per accepted attempt
+                            and per rejected attempt
+
+For the discrete orientation model the transcendental calls (sin, exp and
+the pow(sigma, 1.5), pow(err, -0.2) of the rates and the step control) are
+not counted, pow(x, 2) counts as one multiply and pow(x, 0.5) of a norm as a
+sqrt; each derivative evaluation counts the particle constants (sigma,
+tau_N) that the reference recomputes there, as written.
 """
 import math
 
@@ -329,3 +340,137 @@ if __name__ == "__main__":
     print("heun FLOP per particle-step", heun_flops())
     pi, po = implicit_flops()
     print("implicit FLOP per Newton iteration", pi, "per step besides", po)
+
+
+# ---------------------------------------------------------------- discrete orientation model
+KB = 1.38064852e-23      # include/constants.hpp:10
+GAMMA = 1.76086e11       # include/constants.hpp:12
+
+
+def cpow(x, e):
+    """pow as the step arithmetic uses it: a square is one multiply, a square
+    root of a norm a sqrt, any other exponent a math function (not counted)."""
+    if e == 2:
+        return x * x
+    if e == 0.5:
+        return csqrt(x)
+    return Count(math.pow(float(x), e))
+
+
+def field_sinusoidal(t, h, f):
+    """orc_field_sinusoidal (lib/field.cpp:37-40): h sin(2 pi f t)."""
+    return h * Count(math.sin(float(Count(2 * math.pi) * f * t)))
+
+
+def dom_transition_matrix(k, v, T, h, ms, alpha):
+    """orc_dom_transition_matrix (lib/dom.cpp:30-56)."""
+    sigma = k * v / KB / T
+    taun = v * ms * (1 + alpha * alpha) / 2.0 / GAMMA / alpha / KB / T
+    e1 = sigma * (1 - h) * (1 - h)
+    e2 = sigma * (1 + h) * (1 + h)
+    pre = taun * math.sqrt(math.pi) / cpow(sigma, 1.5) / (1 - h * h)
+    r1 = 1.0 / pre * (1 - h) * Count(math.exp(-float(e1)))
+    r2 = 1.0 / pre * (1 + h) * Count(math.exp(-float(e2)))
+    return [-r2, r1, r2, -r1]
+
+
+def master_equation(W, x):
+    """orc_master_equation (lib/dom.cpp:59-77), dim 2."""
+    out = []
+    for i in range(2):
+        acc = Count(0.0)
+        for j in range(2):
+            acc = acc + W[i * 2 + j] * x[j]
+        out.append(acc)
+    return out
+
+
+def dom_derivs(prm, x, t):
+    """orc_dom_master (lib/dom.cpp:80-93) with the sine field."""
+    k, v, T, ms, alpha, h, f = prm
+    return master_equation(dom_transition_matrix(k, v, T, field_sinusoidal(t, h, f), ms, alpha),
+                           x)
+
+
+CK = dict(C11=0.2, C21=3.0 / 40.0, C22=9.0 / 40.0, C31=3.0 / 10.0, C32=-9.0 / 10.0, C33=6.0 / 5.0,
+          C41=-11.0 / 54.0, C42=2.5, C43=-70.0 / 27.0, C44=35.0 / 27.0, C51=1631.0 / 55296.0,
+          C52=175.0 / 512.0, C53=575.0 / 13824.0, C54=44275.0 / 110592.0, C55=253.0 / 4096.0,
+          H1=0.2, H2=0.3, H3=0.6, H4=1.0, H5=7.0 / 8.0, X11=37.0 / 378.0, X13=250.0 / 621.0,
+          X14=125.0 / 594.0, X16=512.0 / 1771.0, X21=2825.0 / 27648.0, X23=18575.0 / 48384.0,
+          X24=13525.0 / 55296.0, X25=277.0 / 14336.0, X26=0.25)
+
+
+def rk45_step(derivs, x, t, h, tol):
+    """orc_rk45 (lib/integrators.cpp:152-251, Cash-Karp tableau of
+    include/integrators.hpp:128-160) on Count numbers, n = 2.  Returns
+    (next_state, t_new, h_new, flops of the accepted attempt, flops of each
+    rejected attempt)."""
+    c = CK
+    n = len(x)
+    rejected = []
+    while True:
+        f0 = Count.flops
+        k1 = derivs(x, t)
+        y = [k1[i] * h * c["C11"] + x[i] for i in range(n)]
+        k2 = derivs(y, t + h * c["H1"])
+        y = [x[i] + h * (c["C21"] * k1[i] + c["C22"] * k2[i]) for i in range(n)]
+        k3 = derivs(y, t + h * c["H2"])
+        y = [x[i] + h * (c["C31"] * k1[i] + c["C32"] * k2[i] + c["C33"] * k3[i])
+             for i in range(n)]
+        k4 = derivs(y, t + h * c["H3"])
+        y = [x[i] + h * (c["C41"] * k1[i] + c["C42"] * k2[i] + c["C43"] * k3[i] +
+                         c["C44"] * k4[i]) for i in range(n)]
+        k5 = derivs(y, t + h * c["H4"])
+        y = [x[i] + h * (c["C51"] * k1[i] + c["C52"] * k2[i] + c["C53"] * k3[i] +
+                         c["C54"] * k4[i] + c["C55"] * k5[i]) for i in range(n)]
+        k6 = derivs(y, t + h * c["H5"])
+        tmp = [x[i] + h * (c["X11"] * k1[i] + c["X13"] * k3[i] + c["X14"] * k4[i] +
+                           c["X16"] * k6[i]) for i in range(n)]
+        nxt = [x[i] + h * (c["X21"] * k1[i] + c["X23"] * k3[i] + c["X24"] * k4[i] +
+                           c["X25"] * k5[i] + c["X26"] * k6[i]) for i in range(n)]
+        err = Count(0.0)
+        mag = Count(0.0)
+        for i in range(n):
+            mag = mag + x[i] * x[i]
+        mag = cpow(mag, 0.5)
+        for i in range(n):
+            err = err + cpow(abs(tmp[i] - nxt[i]), 2)
+        err = cpow(err, 0.5)
+        err = err / (Count(float(n)) * tol * (1 + mag))
+        if err < 1.0:
+            break
+        f = 0.84 * cpow(err, -0.2)
+        f = Count(0.1) if abs(f) < 0.1 else f
+        h = h * f
+        rejected.append(Count.flops - f0)
+    t_new = t + h
+    f = Count(5.0) if err == 0.0 else 0.84 * cpow(err, -0.2)
+    f = Count(5.0) if f > 5 else f
+    h_new = f * h
+    return nxt, t_new, h_new, Count.flops - f0, rejected
+
+
+def dom_params():
+    """A 10 nm particle of the AC-field workload: K = 23 kJ/m^3, Ms = 400 kA/m,
+    alpha = 0.1, T = 300 K, reduced field amplitude 0.174, 300 kHz."""
+    v = 4.0 / 3.0 * math.pi * (10e-9) ** 3
+    return (Count(23e3), Count(v), Count(300.0), Count(4e5), Count(0.1), Count(0.174),
+            Count(3e5))
+
+
+def dom_flops():
+    """(FLOP of an accepted rk45 attempt, FLOP of a rejected one) on the
+    master equation, counted as the oracle's arithmetic is written."""
+    prm = dom_params()
+    x = [Count(0.7), Count(0.3)]
+    reset()
+    # accepted at once with a non-zero error estimate (the usual case; an
+    # exactly zero estimate skips one multiply)
+    _, _, _, acc, rej = rk45_step(lambda xx, tt: dom_derivs(prm, xx, tt), x, Count(1e-7),
+                                  Count(1e-7), Count(1e-9))
+    assert not rej
+    reset()
+    # a step far too long for the tolerance is rejected first
+    _, _, _, _, rej = rk45_step(lambda xx, tt: dom_derivs(prm, xx, tt), x, Count(1e-7),
+                                Count(1e-3), Count(1e-12))
+    return acc, rej[0]

@@ -9,7 +9,10 @@ field, drift, diffusion and Jacobian functions, with the same Newton
 iteration count; (2) the Heun count equals the 278 FLOP per particle-step
 derived by hand from the reference's formulas (field 13, drift 42, diffusion
 39 per stage, predictor 33, corrector 57), so the counting convention is
-the one the headline roofline uses.
+the one the headline roofline uses.  (3) The counted Cash-Karp step of the
+discrete orientation model reproduces orc_rk45 (lib/integrators.cpp:152-251)
+on the compiled oracle's master equation value for value, rejected attempts
+included.
 """
 import ctypes as C
 import math
@@ -124,7 +127,39 @@ def test_counting_sees_each_operation():
     assert math.isclose(float(c), 1.5 * 2.0 + 3.0 - 0.5)
 
 
+def oracle_dom_ode(prm):
+    """the master equation through the compiled oracle's field, transition
+    matrix and matrix-vector product (lib/dom.cpp:30-93)"""
+    k, v, T, ms, al, h, f = (float(x) for x in prm)
+
+    def ode(x, t):
+        W = np.zeros(4)
+        L.orc_dom_transition_matrix(ptr(W), k, v, T, L.orc_field_sinusoidal(t, h, f), ms, al)
+        dx = np.zeros(2)
+        L.orc_master_equation(ptr(dx), ptr(W), ptr(O.f64(x)), 2)
+        return dx
+    return ode
+
+
+@pytest.mark.parametrize("h,tol", [(1e-7, 1e-9), (1e-5, 1e-9), (1e-3, 1e-12), (3e-8, 1e-10)])
+def test_counted_rk45_step_is_the_oracle_step(h, tol):
+    prm = F.dom_params()
+    x = [0.7, 0.3]
+    t = 1e-7
+    nxt, _, t_new, h_new = O.rk45(oracle_dom_ode(prm), x, t, h, tol)
+    F.reset()
+    cn, ct, ch, acc, rej = F.rk45_step(lambda xx, tt: F.dom_derivs(prm, xx, tt),
+                                       [F.Count(v) for v in x], F.Count(t), F.Count(h),
+                                       F.Count(tol))
+    assert np.array_equal([float(v) for v in cn], nxt)
+    assert float(ct) == t_new and float(ch) == h_new
+    # each attempt costs six derivative evaluations of 44 FLOP and the
+    # stage and error arithmetic around them
+    assert acc == 401 and all(r == 400 for r in rej)
+
+
 def test_bench_constants_are_the_counts():
     import bench
     assert bench.FLOP_PER_PARTICLE_STEP == F.heun_flops()
     assert (bench.FLOP_IMPLICIT_PER_NEWTON, bench.FLOP_IMPLICIT_PER_STEP) == F.implicit_flops()
+    assert bench.FLOP_DOM_PER_STEP == F.dom_flops()[0]
