@@ -111,8 +111,14 @@ __device__ inline bool dom_rk45(const DomArgs &a, double k, double v, double tem
             next[i] = x[i] + h * (2825.0 / 27648.0 * k1[i] + 18575.0 / 48384.0 * k3[i] +
                                   13525.0 / 55296.0 * k4[i] + 277.0 / 14336.0 * k5[i] +
                                   0.25 * k6[i]);
-        const double mag = pow(x[0] * x[0] + x[1] * x[1], 0.5);
-        err = pow(pow(fabs(x5[0] - next[0]), 2) + pow(fabs(x5[1] - next[1]), 2), 0.5);
+        // the reference's pow(., 2) and pow(., 0.5) of these norms as a square
+        // and a square root: x * x is pow(x, 2) exactly, and sqrt, correctly
+        // rounded, is what an accurate pow(x, 0.5) returns; a general FP64
+        // pow is a long dependent chain, and these were five of the six per
+        // attempt
+        const double mag = sqrt(x[0] * x[0] + x[1] * x[1]);
+        const double d0 = fabs(x5[0] - next[0]), d1 = fabs(x5[1] - next[1]);
+        err = sqrt(d0 * d0 + d1 * d1);
         err /= (2.0 * tol * (1 + mag));
         if (err < 1.0) {
             ok = true;
