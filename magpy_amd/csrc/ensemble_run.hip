@@ -67,41 +67,54 @@ std::vector<T> cast_vec(const std::vector<double> &v)
     return std::vector<T>(v.begin(), v.end());
 }
 
-// Whether Heun over n_waves integrating waves runs on the two-wave
-// kernel (a normal-generating wave beside each integrating wave): in FP32 it
-// pays while the integrating waves are at most one per SIMD of the current
-// device; in FP64 the one-wave kernel is faster (65,536 particles: 5.7e10
-// against 4.3e10 particle-steps/s).  MAGPY_AMD_HEUN_TWO_WAVE=0/1 forces the
-// choice in either precision.
-bool two_wave_pays(int n_waves, bool fp32)
+// Which Heun kernel runs n_waves integrating waves: 0 the one-wave kernel,
+// 1 the two-wave kernel with the normals formed by the producing wave, 2 (FP32)
+// the two-wave kernel with Box-Muller in the integrating wave.  FP32: 2 while
+// the integrating waves are more than one per CU and at most one per SIMD, 1
+// at one per CU or fewer, else 0 (32,768-step launches: 65,536 particles
+// 3.56e11 / 3.18e11 / 2.71e11 for 2 / 1 / 0; 16,384: 0.91e11 / 1.20e11 /
+// 0.68e11; 131,072: 3.17e11 / 2.85e11 / 3.62e11).  FP64: 0, the one-wave
+// kernel is faster (65,536 particles: 5.7e10 against 4.3e10).
+// MAGPY_AMD_HEUN_TWO_WAVE=0/1/2 forces the choice (2 means 1 in FP64).
+int two_wave_mode(int n_waves, bool fp32)
 {
-    if (const char *env = std::getenv("MAGPY_AMD_HEUN_TWO_WAVE")) return std::atoi(env) != 0;
-    if (!fp32) return false;
+    if (const char *env = std::getenv("MAGPY_AMD_HEUN_TWO_WAVE")) {
+        const int v = std::atoi(env);
+        return v <= 0 ? 0 : (v >= 2 && fp32) ? 2 : 1;
+    }
+    if (!fp32) return 0;
     int dev = 0, cus = 0;
     MP_HIP(hipGetDevice(&dev));
     MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    return n_waves <= 4 * cus;
+    if (n_waves > 4 * cus) return 0;
+    return n_waves > cus ? 2 : 1;
 }
 
 // grid: blocks of kBlock lanes (n_waves = 4 grid integrating waves)
 template <typename T, int kShape, bool kArray, bool kRenorm>
 void launch_heun(const mp::IndepArgs<T> &a, int grid, hipStream_t s, const float *htab,
-                 bool two_wave)
+                 int two_wave)
 {
     if constexpr (!kArray) {
-        if (two_wave) {
+        if (two_wave != 0) {
             // one (integrating, producing) wave pair per workgroup; two or
             // four pairs per workgroup measure the same or slower
             const dim3 g2((unsigned)(grid * (mp::kBlock / mp::kWave))), b2(2 * mp::kWave);
             if constexpr (sizeof(T) == 4) {
+                auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, g2, b2, 0, s, a, htab); };
                 if (htab) {
-                    hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, mp::kConstant, true, kRenorm>),
-                                       g2, b2, 0, s, a, htab);
+                    if (two_wave == 2)
+                        go(mp::heun_indep_pc_kernel<T, mp::kConstant, true, kRenorm, true>);
+                    else
+                        go(mp::heun_indep_pc_kernel<T, mp::kConstant, true, kRenorm, false>);
                     return;
                 }
+                if (two_wave == 2) go(mp::heun_indep_pc_kernel<T, kShape, false, kRenorm, true>);
+                else go(mp::heun_indep_pc_kernel<T, kShape, false, kRenorm, false>);
+                return;
             }
-            hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, kShape, false, kRenorm>), g2, b2, 0, s,
-                               a, htab);
+            hipLaunchKernelGGL((mp::heun_indep_pc_kernel<T, kShape, false, kRenorm, false>), g2,
+                               b2, 0, s, a, htab);
             return;
         }
     }
@@ -119,7 +132,7 @@ void launch_heun(const mp::IndepArgs<T> &a, int grid, hipStream_t s, const float
 template <typename T, int kShape, bool kArray>
 void launch_step(const mp::IndepArgs<T> &a, int scheme, double eps, int max_iter,
                  unsigned long long *it, unsigned long long *fl, int grid, hipStream_t s,
-                 const float *htab, bool two_wave)
+                 const float *htab, int two_wave)
 {
     if constexpr (sizeof(T) == 8) {
         if (scheme == MP_SCHEME_IMPLICIT_MIDPOINT) {
@@ -138,7 +151,7 @@ void launch_step(const mp::IndepArgs<T> &a, int scheme, double eps, int max_iter
 template <typename T, bool kArray>
 void launch_shape(int shape_kind, const mp::IndepArgs<T> &a, int scheme, double eps,
                   int max_iter, unsigned long long *it, unsigned long long *fl, int grid,
-                  hipStream_t s, const float *htab, bool tw)
+                  hipStream_t s, const float *htab, int tw)
 {
     switch (shape_kind) {
     case mp::kSine: launch_step<T, mp::kSine, kArray>(a, scheme, eps, max_iter, it, fl, grid, s, htab, tw); break;
@@ -237,7 +250,7 @@ struct GroupRun : GroupBase {
     int max_iter = 1000;
     double eps = 1e-9;
     bool array_noise = false, want_traj = false, want_ens = false, cluster = false;
-    bool two_wave = false;     // Heun on heun_indep_pc_kernel
+    int two_wave = 0;          // Heun kernel: 0 one wave, 1/2 two waves (two_wave_mode)
     int lanes = 1;             // lanes per cluster slot (1 or 2)
     int64_t S = 0, s_next = 0, spl = 1;
     std::vector<int64_t> sched, slot_particle;
@@ -388,8 +401,9 @@ struct GroupRun : GroupBase {
         // cluster slots come in whole blocks of block / lanes
         grid = cluster ? n / (block / lanes) : (n + block - 1) / block;
         n_waves = (grid * block) / mp::kWave;
-        two_wave = !cluster && !array_noise && scheme == MP_SCHEME_HEUN &&
-                   two_wave_pays(n_waves, sizeof(T) == 4);
+        two_wave = !cluster && !array_noise && scheme == MP_SCHEME_HEUN
+                       ? two_wave_mode(n_waves, sizeof(T) == 4)
+                       : 0;
         if (want_traj) d_samples.alloc((size_t)S * 3 * (size_t)n);
         if (want_ens) {
             // each recorded sample writes all of its (component, wave) slots;

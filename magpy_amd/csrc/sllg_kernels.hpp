@@ -425,11 +425,19 @@ struct HeunRing {
 // vector loads, which complete in order like its LDS reads, so the reads of
 // the next group can be in flight while a group is integrated (scalar loads
 // complete out of order and would make every wait a wait for all of them).
-template <typename T, int kShape, bool kTable, bool kRenorm>
+// kBmInt (FP32): the producing wave writes the group's Philox words and the
+// integrating wave forms the normals (Box-Muller); otherwise the producer
+// writes the normals.  With one integrating wave per SIMD the SIMD pairs an
+// integrating wave with another workgroup's producing wave, and moving the
+// Box-Muller work to the integrating wave fills the stalls of its step chain;
+// with fewer integrating waves than SIMDs the integrating wave runs alone and
+// the producer should carry it.
+template <typename T, int kShape, bool kTable, bool kRenorm, bool kBmInt>
 __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a,
                                                                   const float *htab)
 {
     static_assert(sizeof(T) == 4 || !kTable, "the shared field table is FP32");
+    static_assert(sizeof(T) == 4 || !kBmInt, "Box-Muller in the integrating wave is FP32");
     constexpr int K = HeunRing<T, kTable>::kGroups, R = HeunRing<T, kTable>::kRows;
     __shared__ T ring[2][K][R][kWave];
     const int lane = threadIdx.x & (kWave - 1);
@@ -455,6 +463,7 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
     const T k = a.k[q], al = a.alpha[q], dt = a.dt[q];
     const T kex = k * ex, key = k * ey, kez = k * ez, edx = ex * dt, edy = ey * dt, edz = ez * dt;
     const double msw = a.msw[q];
+    const float cbm = Num<float>::kRadius2 * (float)(sig * sig);   // FP32 Box-Muller radius scale
     int64_t s = a.s_next;
     int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
     // the integrator takes issue priority over co-resident waves
@@ -467,10 +476,25 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
                 for (int kk = 0; kk < K; ++kk) {
                     const int64_t g = g_begin + c * K + kk;
                     if (g >= g_end) break;
-                    T z[12];
-                    group_normals(g, pid, a.tag, keys, sig, z);
+                    if constexpr (kBmInt) {
+                        // the group's twelve Philox words (as float bit
+                        // patterns); the integrating wave forms the normals
 #pragma unroll
-                    for (int i = 0; i < 12; ++i) half[kk][i][lane] = z[i];
+                        for (int i = 0; i < 3; ++i) {
+                            const uint64_t cn = (uint64_t)(3 * g + i);
+                            const u32x4 w = philox4x32_10(
+                                u32x4{(uint32_t)cn, (uint32_t)(cn >> 32), pid, a.tag}, keys);
+                            half[kk][4 * i][lane] = __uint_as_float(w.x);
+                            half[kk][4 * i + 1][lane] = __uint_as_float(w.y);
+                            half[kk][4 * i + 2][lane] = __uint_as_float(w.z);
+                            half[kk][4 * i + 3][lane] = __uint_as_float(w.w);
+                        }
+                    } else {
+                        T z[12];
+                        group_normals(g, pid, a.tag, keys, sig, z);
+#pragma unroll
+                        for (int i = 0; i < 12; ++i) half[kk][i][lane] = z[i];
+                    }
                     if constexpr (!kTable) {
                         const double base = (double)(4 * g);
 #pragma unroll
@@ -526,8 +550,19 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
                 const int64_t g = gc + kk;
                 if (g >= g_end) break;
                 T z[12], h1[4], h2[4];
+                if constexpr (kBmInt) {
+                    // Box-Muller of the producer's words (group_normals' pairs
+                    // and expressions): independent work of this wave that
+                    // fills the stalls of its step chain
 #pragma unroll
-                for (int i = 0; i < 12; ++i) z[i] = zn[i];
+                    for (int i = 0; i < 6; ++i)
+                        Num<float>::box_muller_r2(__float_as_uint(zn[2 * i]),
+                                                  __float_as_uint(zn[2 * i + 1]), cbm, z[2 * i],
+                                                  z[2 * i + 1]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) z[i] = zn[i];
+                }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     h1[u] = h1n[u];

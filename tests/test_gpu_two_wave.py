@@ -1,15 +1,16 @@
-"""The two-wave Heun kernel equals the one-wave kernel bit for bit.
+"""The two-wave Heun kernels equal the one-wave kernel bit for bit.
 
 For ensembles of at most one integrating wave per SIMD the library runs
-heun_indep_pc_kernel (one wave generating the normals and field values into
-LDS, one integrating); larger ensembles run heun_indep_kernel.  Both compute
-every value with the same expressions, so forcing either kernel
-(MAGPY_AMD_HEUN_TWO_WAVE) on the same ensemble must give identical samples,
-final states and ensemble sums: with a shared field table (every member on
-one field, FP32), with per-lane fields (members on different frequencies
-and shapes), with samples inside 4-step groups, launch blocks that do not end
-on a group, a particle count that is not a multiple of 64, renormalisation,
-in FP32 and FP64.
+heun_indep_pc_kernel (one wave producing the noise into LDS, one
+integrating): with the producer forming the normals (mode 1), or, in FP32,
+writing the Philox words for the integrating wave to form them (mode 2);
+larger ensembles run heun_indep_kernel (mode 0).  All compute every value
+with the same expressions, so forcing each kernel (MAGPY_AMD_HEUN_TWO_WAVE)
+on the same ensemble must give identical samples, final states and ensemble
+sums: with a shared field table (every member on one field, FP32), with
+per-lane fields (members on different frequencies and shapes), with samples
+inside 4-step groups, launch blocks that do not end on a group, a particle
+count that is not a multiple of 64, renormalisation, in FP32 and FP64.
 """
 import numpy as np
 import pytest
@@ -37,12 +38,12 @@ def ensemble(P, seed, mixed):
     return e
 
 
-def run_both(monkeypatch, precision, **kw):
+def run_modes(monkeypatch, precision, **kw):
     out = {}
-    for tw in ("0", "1"):
+    for tw in ("0", "1", "2"):
         monkeypatch.setenv("MAGPY_AMD_HEUN_TWO_WAVE", tw)
         out[tw] = core.simulate_ensemble(precision=precision, **kw)
-    return out["0"], out["1"]
+    return out
 
 
 @pytest.mark.parametrize("precision,P,mixed,renorm,block", [
@@ -56,11 +57,15 @@ def test_two_wave_equals_one_wave(monkeypatch, precision, P, mixed, renorm, bloc
     e = ensemble(P, 17 + P, mixed)
     dt = 1e-14
     # 4,003 steps; 37 samples fall at steps that are not multiples of 4
-    one, two = run_both(monkeypatch, precision, time_step=dt, end_time=4002.5 * dt,
-                        max_samples=37, renorm=renorm, **e)
-    assert one["n_steps"] == two["n_steps"]
-    assert np.all(one["status"] == 0) and np.all(two["status"] == 0)
-    for c in "xyz":
-        assert np.array_equal(one[c], two[c]), c
-    assert np.array_equal(one["final_state"], two["final_state"])
-    assert np.array_equal(one["ensemble_m"], two["ensemble_m"])
+    out = run_modes(monkeypatch, precision, time_step=dt, end_time=4002.5 * dt,
+                    max_samples=37, renorm=renorm, **e)
+    one = out["0"]
+    assert np.all(one["status"] == 0)
+    for mode in ("1", "2"):
+        two = out[mode]
+        assert one["n_steps"] == two["n_steps"]
+        assert np.all(two["status"] == 0)
+        for c in "xyz":
+            assert np.array_equal(one[c], two[c]), (mode, c)
+        assert np.array_equal(one["final_state"], two["final_state"]), mode
+        assert np.array_equal(one["ensemble_m"], two["ensemble_m"]), mode
