@@ -518,10 +518,9 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
         } else if (c > 0) {
             T (*half)[R][kWave] = ring[(c - 1) & 1];
             const int64_t gc = g_begin + (c - 1) * K;   // first group of the chunk (< g_end)
-            // group kk's normals and field values; loaded one group ahead so
-            // the LDS and scalar-load latency overlaps the previous group's steps
-            // (a group past the end of the block reads the last group's table
-            // entries and ring rows it does not use)
+            // group kk's ring rows and field values (a group past the end of
+            // the block reads the last group's table entries and ring rows it
+            // does not use)
             auto load_group = [&](int kk, T *zz, T *hh1, T *hh2) {
 #pragma unroll
                 for (int i = 0; i < 12; ++i) zz[i] = half[kk][i][lane];
@@ -541,47 +540,50 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
                     }
                 }
             };
-            T zn[12], h1n[4], h2n[4];
-            load_group(0, zn, h1n, h2n);
-            // unrolled, so the prefetched registers need no copies (a copy of
-            // a scalar load's result would wait for it at once)
+            // ring words and field values of every group of the chunk, loaded
+            // two groups ahead; the normals of group kk + 1 are formed in the
+            // same basic block as the steps of group kk, so their independent
+            // work fills the stalls of the step chain.  Unrolled: every index
+            // is a constant and the arrays stay in registers.
+            T raw[K][12], h1a[K][4], h2a[K][4], zb[K][12];
+            auto normals = [&](int kk) {
+                if constexpr (kBmInt) {
+                    // Box-Muller of the producer's words (group_normals' pairs
+                    // and expressions)
+#pragma unroll
+                    for (int i = 0; i < 6; ++i)
+                        Num<float>::box_muller_r2(__float_as_uint(raw[kk][2 * i]),
+                                                  __float_as_uint(raw[kk][2 * i + 1]), cbm,
+                                                  zb[kk][2 * i], zb[kk][2 * i + 1]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 12; ++i) zb[kk][i] = raw[kk][i];
+                }
+            };
+            load_group(0, raw[0], h1a[0], h2a[0]);
+            if constexpr (K > 1) load_group(1, raw[1], h1a[1], h2a[1]);
+            normals(0);
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) {
                 const int64_t g = gc + kk;
                 if (g >= g_end) break;
-                T z[12], h1[4], h2[4];
-                if constexpr (kBmInt) {
-                    // Box-Muller of the producer's words (group_normals' pairs
-                    // and expressions): independent work of this wave that
-                    // fills the stalls of its step chain
-#pragma unroll
-                    for (int i = 0; i < 6; ++i)
-                        Num<float>::box_muller_r2(__float_as_uint(zn[2 * i]),
-                                                  __float_as_uint(zn[2 * i + 1]), cbm, z[2 * i],
-                                                  z[2 * i + 1]);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 12; ++i) z[i] = zn[i];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    h1[u] = h1n[u];
-                    h2[u] = h2n[u];
-                }
-                if (kk + 1 < K) load_group(kk + 1, zn, h1n, h2n);
+                if (kk + 2 < K) load_group(kk + 2, raw[kk + 2], h1a[kk + 2], h2a[kk + 2]);
                 const bool fast = 4 * g >= n0 && 4 * g + 4 <= n1 && next_rec > 4 * g + 4;
                 if (fast) {
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
-                        heun_update<T, kRenorm>(mx, my, mz, z[3 * u], z[3 * u + 1], z[3 * u + 2],
-                                                h1[u], h2[u], kex, key, kez, edx, edy, edz, dt, al);
+                        heun_update<T, kRenorm>(mx, my, mz, zb[kk][3 * u], zb[kk][3 * u + 1],
+                                                zb[kk][3 * u + 2], h1a[kk][u], h2a[kk][u], kex, key,
+                                                kez, edx, edy, edz, dt, al);
+                    if (kk + 1 < K) normals(kk + 1);
                 } else {
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const int64_t n = 4 * g + u;
                         if (n < n0 || n >= n1) continue;
-                        heun_update<T, kRenorm>(mx, my, mz, z[3 * u], z[3 * u + 1], z[3 * u + 2],
-                                                h1[u], h2[u], kex, key, kez, edx, edy, edz, dt, al);
+                        heun_update<T, kRenorm>(mx, my, mz, zb[kk][3 * u], zb[kk][3 * u + 1],
+                                                zb[kk][3 * u + 2], h1a[kk][u], h2a[kk][u], kex,
+                                                key, kez, edx, edy, edz, dt, al);
                         if (n + 1 == next_rec) {
                             do {
                                 record_sample(a, s, p, live, mx, my, mz, msw, wave);
@@ -590,6 +592,7 @@ __global__ __launch_bounds__(2 * kWave) void heun_indep_pc_kernel(IndepArgs<T> a
                             } while (next_rec == n + 1);
                         }
                     }
+                    if (kk + 1 < K) normals(kk + 1);
                 }
             }
         }
