@@ -73,7 +73,7 @@ PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks (published)
 # HBM bytes of one default launch (65,536 particles x 131,072 steps), from
 # rocprofv3 FETCH_SIZE (x2.000) + WRITE_SIZE (x1.000) scaled against a 1 GiB
 # device copy; see profiles/heun_indep_pc_fp32_65536_ac_field.md.
-TRAFFIC_BYTES_DEFAULT_LAUNCH = 9.28e6
+TRAFFIC_BYTES_DEFAULT_LAUNCH = 9.26e6
 DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256),
             "replicas": (4096, 64), "adjoint": (16384, 1048000), "dom": (65536, 1)}
 

@@ -72,8 +72,8 @@ std::vector<T> cast_vec(const std::vector<double> &v)
 // the two-wave kernel with Box-Muller in the integrating wave.  FP32: 2 while
 // the integrating waves are more than one per CU and at most one per SIMD, 1
 // at one per CU or fewer, else 0 (32,768-step launches: 65,536 particles
-// 3.56e11 / 3.18e11 / 2.71e11 for 2 / 1 / 0; 16,384: 0.91e11 / 1.20e11 /
-// 0.68e11; 131,072: 3.17e11 / 2.85e11 / 3.62e11).  FP64: 0, the one-wave
+// 3.63e11 / 3.02e11 / 2.72e11 for 2 / 1 / 0; 16,384: 0.94e11 / 1.21e11 /
+// 0.68e11; 131,072: 2.99e11 / 2.82e11 / 3.66e11).  FP64: 0, the one-wave
 // kernel is faster (65,536 particles: 5.7e10 against 4.3e10).
 // MAGPY_AMD_HEUN_TWO_WAVE=0/1/2 forces the choice (2 means 1 in FP64).
 int two_wave_mode(int n_waves, bool fp32)
