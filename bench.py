@@ -804,7 +804,8 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build's path runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(how the path would shard over several GPUs is a design, not a run)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "adjoint":
