@@ -404,15 +404,16 @@ __global__ __launch_bounds__(kBlock) void heun_indep_kernel(IndepArgs<T> a,
 // ------------------------------------------------------------------ Heun, two waves
 // The step block for independent particles with the work of 64
 // particles split over a workgroup of two waves: wave 1 (producer) generates
-// the scaled Wiener normals of 4-step groups and, without a shared field
-// table, the applied field at both stage times of each step, into an LDS ring
-// of two halves of kGroups groups; wave 0 (integrator) runs the Heun steps
-// from the other half; the two swap halves at a barrier.  The integrator's
-// dependency chain then carries only the Heun arithmetic, and the producer's
-// independent work fills the SIMD cycles the chain leaves idle, which pays
-// where the particles are too few to give each SIMD several integrating
-// waves.  Every value is computed by the same expression as in
-// heun_indep_kernel, so the results are the same bit for bit.
+// the noise of 4-step groups — the scaled Wiener normals, or with kBmInt the
+// Philox words they are formed from — and, without a shared field table, the
+// applied field at both stage times of each step, into an LDS ring of two
+// halves of kGroups groups; wave 0 (integrator) runs the Heun steps from the
+// other half; the two swap halves at a barrier.  The integrator's dependency
+// chain then carries little besides the Heun arithmetic, and independent work
+// fills the SIMD cycles the chain leaves idle, which pays where the particles
+// are too few to give each SIMD several integrating waves.  Every value is
+// computed by the same expression as in heun_indep_kernel, so the results
+// are the same bit for bit.
 // Ring halves of 4 groups (24 KB, FP32 with a table), 2 groups (20 KB, FP32
 // with per-lane fields) or 1 group (20 KB, FP64): four workgroups of a CU fit.
 template <typename T, bool kTable>
