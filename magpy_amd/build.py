@@ -1,6 +1,7 @@
 """Build recipe of the magpy_amd native code (run from anywhere).
 
-  python -m magpy_amd.build        # or magpy_amd.build.build()
+  python magpy_amd/build.py        # run by path: the package itself imports
+                                   # the binding this script builds
 
 1. libmagpy_amd.so   hipcc, gfx950, from csrc/ensemble_run.hip and
                      csrc/adjoint.hip and csrc/dom.hip (kernels in

@@ -132,6 +132,8 @@ def lib():
                                                                   C.c_double, C.c_size_t, _dp,
                                                                   C.c_double, C.c_double,
                                                                   C.c_size_t, _dp, _dp, _dp]
+        L.orc_dnrm2.restype = C.c_double
+        L.orc_dnrm2.argtypes = [C.c_int, _dp]
         L.orc_sample_schedule.restype = C.c_int64
         L.orc_sample_schedule.argtypes = [C.c_double, C.c_double, C.c_size_t, _i64p]
         L.orc_full_dynamics.restype = C.c_int
@@ -216,6 +218,11 @@ def step_wiener(seed, n_particles, step, tag=0):
 
 
 # ---------------------------------------------------------------- schedule
+def dnrm2(x):
+    x = f64(x)
+    return lib().orc_dnrm2(x.size, ptr(x))
+
+
 def sample_schedule(dt_reduced, end_reduced, max_samples):
     k = np.zeros(max_samples, dtype=np.int64)
     n_steps = lib().orc_sample_schedule(dt_reduced, end_reduced, max_samples, ptr(k))

@@ -8,9 +8,12 @@
  * library is built with -ffp-contract=off for the same reason.
  *
  * Pinning status of each part is recorded in the test suite
- * (tests/test_oracle_*.py); the one part with no pin is noted inline:
- *   - orc_dnrm2 (BLAS dnrm2 restated as a plain 2-norm): parity unpinned
- *     beyond the closed form.
+ * (tests/test_oracle_*.py).  orc_dnrm2 is pinned against Python's math.hypot
+ * (error below one ulp) on ordinary, huge and tiny vectors; the
+ * full_dynamics orchestration (unit reduction and time factor, the field's
+ * time convention, the zero-order-hold sample index, the SI conversion) is
+ * pinned by closed-form solutions at zero temperature
+ * (tests/test_oracle_closed_forms.py).
  */
 #include "sllg_oracle.h"
 
@@ -464,13 +467,17 @@ void orc_heun(double *next, double *drift, double *tdrift, double *diff, double 
     }
 }
 
-/* BLAS dnrm2 as a plain 2-norm (parity unpinned beyond the closed form:
- * BLAS libraries scale to avoid overflow, which changes the last bit). */
-static double orc_dnrm2(int n, const double *x)
+/* BLAS dnrm2 (cblas_dnrm2 at lib/optimisation.cpp:98,142 and
+ * lib/simulation.cpp:383): the Euclidean norm without overflow or underflow
+ * for any finite double input.  The squares are summed in the x87 80-bit
+ * format (64-bit significand, 15-bit exponent), whose range holds the square
+ * of every double, and the root is rounded once more to double: the result
+ * is within one ulp of the exact norm, as BLAS implementations are. */
+double orc_dnrm2(int n, const double *x)
 {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += x[i] * x[i];
-    return sqrt(s);
+    long double s = 0.0L;
+    for (int i = 0; i < n; ++i) s += (long double)x[i] * (long double)x[i];
+    return (double)sqrtl(s);
 }
 
 /* LAPACK dgesv semantics for a row-major A (LAPACKE_dgesv_work, called at

@@ -175,6 +175,9 @@ int orc_full_dynamics(
     int rng_kind, const double *noise, size_t noise_len,
     double *time, double *field, double *mx, double *my, double *mz, orc_stats *stats);
 
+/* BLAS dnrm2: Euclidean norm of x[0..n), overflow-free, within one ulp. */
+double orc_dnrm2(int n, const double *x);
+
 /* Sample schedule of the reduced loop (lib/simulation.cpp:342-406): k[s] is
  * the step index whose state is recorded as sample s; returns the number of
  * integration steps the loop takes (k[S-1] + 1). */
