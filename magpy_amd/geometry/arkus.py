@@ -5,7 +5,7 @@ particle centres at unit contact distance: the minimum-energy sticky-sphere
 clusters of N. Arkus, V. N. Manoharan and M. P. Brenner, Phys. Rev. Lett.
 103, 118303 (2009), for 1-8 particles (9 has no entries), in the
 reference's ordering and orientation.  The numbers live in arkus.json
-(written by make_arkus_table.py).
+(written by tests/golden/make_arkus_table.py).
 """
 import json
 import os

@@ -40,20 +40,19 @@ class Results:
         self.N = N
 
     def plot(self):
-        """x, y, z of every particle against time (magpy/results.py:39-59)."""
+        """One panel per particle showing its x, y and z magnetisation
+        against time (magpy/results.py:39-59); returns the figure."""
         import matplotlib.pyplot as plt  # optional dependency, as in the reference
-        fg, axs = plt.subplots(nrows=self.N)
-        if self.N == 1:
-            axs = [axs]
-        for idx in range(self.N):
-            axs[idx].plot(self.time, self.x[idx], label='x')
-            axs[idx].plot(self.time, self.y[idx], label='y')
-            axs[idx].plot(self.time, self.z[idx], label='z')
-            axs[idx].legend()
-            axs[idx].set_title('Particle {}'.format(idx))
-            axs[idx].set_xlabel('Reduced time [dimless]')
-            fg.tight_layout()
-        return fg
+        fig, panels = plt.subplots(nrows=self.N, squeeze=False)
+        for idx, ax in enumerate(panels[:, 0]):
+            for comp in ('x', 'y', 'z'):
+                ax.plot(self.time, getattr(self, comp)[idx], label=comp)
+            ax.set_title('Particle %d' % idx)
+            ax.set_xlabel('Time [s]')
+            ax.set_ylabel('Magnetisation [A/m]')
+            ax.legend()
+        fig.tight_layout()
+        return fig
 
     def magnetisation(self, direction='z'):
         """Sum over the cluster's particles of one component (magpy/results.py:61-75)."""

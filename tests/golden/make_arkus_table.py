@@ -7,14 +7,18 @@ orientation; configuration ids must select the same coordinates, so it is
 kept identical.  The reference file is parsed as text with `ast` and only
 its numeric literals are read; nothing from it is executed.
 
-  python -m magpy_amd.geometry.make_arkus_table /root/reference/magpy/geometry/arkus.py
+  python tests/golden/make_arkus_table.py /root/reference/magpy/geometry/arkus.py
+
+It writes magpy_amd/geometry/arkus.json; the shipped package only reads that
+file.
 """
 import ast
 import json
 import os
 import sys
 
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "arkus.json")
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+OUT = os.path.join(ROOT, "magpy_amd", "geometry", "arkus.json")
 
 
 def extract(path):
