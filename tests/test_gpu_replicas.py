@@ -58,6 +58,27 @@ def test_dipolar_product_matches_oracle_pair_sum():
         assert err < t, (mode, err)
 
 
+def test_512_particle_product_matches_oracle_pair_sum():
+    # the shared-geometry workload's cluster: 512 particles at 15 % volume
+    # fraction, 3N = 1536 columns, 48 k stages; 128 replicas make 8 column
+    # tiles of one row block, so blocks are renumbered for the XCDs
+    print("CASE dipolar_product_512", flush=True)
+    rng = np.random.default_rng(21)
+    n, reps = 512, 128
+    loc = G.random_cluster(n, 10e-9, 0.15, 2.2, rng)
+    radius = np.full(n, 10e-9)
+    K = np.full(n, 23e3)
+    m = rng.normal(size=(reps, n, 3))
+    m /= np.linalg.norm(m, axis=2, keepdims=True)
+    ref = np.stack([oracle_field(loc, radius, K, 4e5, m[r]) for r in range(reps)])
+    scale = np.linalg.norm(ref, axis=(1, 2), keepdims=True)
+    for mode, t in {"pairs": 1e-12, "gemm_f32": 2e-6, "gemm_bf16x3": 5e-5}.items():
+        h = core.dipolar_field(loc, radius, K, 4e5, m, mode=mode)
+        err = np.max(np.linalg.norm(h - ref, axis=(1, 2), keepdims=True) / scale)
+        print(mode, "max relative error", err, flush=True)
+        assert err < t, (mode, err)
+
+
 def replica_ensemble(n, reps, seed, phi=0.15):
     rng = np.random.default_rng(seed)
     loc = G.random_cluster(n, 10e-9, phi, 2.2, rng)
@@ -95,6 +116,27 @@ def test_replica_path_matches_pair_sums():
             print(mode, c, err, flush=True)
             assert err < tol, (mode, c, err)
     assert np.all(res["gemm_bf16x3"]["status"] == 0)
+
+
+def test_512_particle_replicas_track_fp64_pair_sums():
+    # 200 steps of 128 replicas of the 512-particle cluster on both products
+    # against the FP64 pair-sum kernel on the same Philox stream
+    print("CASE replica_512_vs_fp64", flush=True)
+    e = replica_ensemble(512, 128, 4)
+    dt, S = 1e-14, 5
+    end = 200 * dt
+    res = {}
+    for mode in ("gemm_f32", "gemm_bf16x3"):
+        ses = run(e, dt, end, S, precision="fp32", dipolar=mode)
+        assert ses.info()["dipolar"] == mode
+        res[mode] = ses.finish()
+    res["fp64"] = run(e, dt, end, S, precision="fp64", dipolar="pairs").finish()
+    for mode in ("gemm_f32", "gemm_bf16x3"):
+        assert np.all(res[mode]["status"] == 0)
+        for c in "xyz":
+            err = np.max(np.abs(res[mode][c] - res["fp64"][c])) / 4e5
+            print(mode, c, err, flush=True)
+            assert err < 1e-5, (mode, c, err)
 
 
 def test_auto_selects_the_product_for_replicas():
