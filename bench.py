@@ -155,11 +155,47 @@ def workload(name, n, timed_steps):
     raise SystemExit("unknown workload " + name)
 
 
+def host_cpu():
+    """The host the CPU baseline runs on: its CPU model, logical CPUs, the
+    CPUs this process may run on (affinity), a cgroup CPU quota if one is
+    set, and the CPU share the job is given (OMP_NUM_THREADS, set by the
+    job's environment).  The baseline uses every CPU of the affinity set
+    within the quota and the share."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    share = os.environ.get("OMP_NUM_THREADS")
+    share = int(share) if share and share.isdigit() and int(share) > 0 else None
+    threads = affinity
+    if quota is not None:
+        threads = min(threads, max(1, int(quota)))
+    if share is not None:
+        threads = min(threads, share)
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": affinity,
+            "cgroup_cpu_quota": quota, "job_cpu_share": share, "threads": threads}
+
+
 def cpu_baseline(name, inp, seconds=12.0):
     """Time the oracle on the host cores (threads release the GIL in ctypes)."""
     from oracle import oracle as O
     O.lib()
-    n_threads = min(os.cpu_count() or 1, 16)
+    host = host_cpu()
+    n_threads = host["threads"]
     offs = inp["member_offsets"]
     n_members = len(offs) - 1
     if name == "ac_field":
@@ -211,7 +247,7 @@ def cpu_baseline(name, inp, seconds=12.0):
     what = {"ac_field": "Heun", "implicit": "implicit-midpoint", "clusters": "Heun, dipolar",
             "replicas": "Heun, dipolar (512 particles)"}[name]
     return {"value": allc, "unit": "particle-steps/s", "cores": n_threads, "kind": "port",
-            "value_1core": one,
+            "value_1core": one, "host": host,
             "sample": "%d members x %d %s steps each (FP64 oracle in the reference's structure: "
                       "mt19937_64 Wiener draws, dense 3N x 3N diffusion, function-pointer sde) "
                       "on %d threads; 1-core rate from %d members" % (ma, steps + 1, what,
@@ -590,7 +626,8 @@ def dom_cpu_baseline(inp, steps_per_particle, seconds=12.0):
     device run's per-particle counts (the same adaptive algorithm)."""
     from oracle import oracle as O
     O.lib()
-    n_threads = min(os.cpu_count() or 1, 16)
+    host = host_cpu()
+    n_threads = host["threads"]
     P = len(inp["volume"])
     lock = threading.Lock()
     state = {"next": 0, "done": 0}
@@ -630,6 +667,7 @@ def dom_cpu_baseline(inp, steps_per_particle, seconds=12.0):
     one, m1 = run(1, seconds / 4)
     allc, ma = run(n_threads, seconds)
     return {"value": allc, "unit": "particle-RK45-steps/s", "cores": n_threads, "kind": "port",
+            "host": host,
             "value_1core": one,
             "sample": "%d particles' whole runs of the FP64 oracle's dom_ensemble_dynamics "
                       "(function-pointer master equation, reference rk45) on %d threads; "
@@ -685,7 +723,8 @@ def adjoint_cpu_baseline(inp, theta, seconds=12.0):
     from oracle import oracle as O
     import magpy_amd
     O.lib()
-    n_threads = min(os.cpu_count() or 1, 16)
+    host = host_cpu()
+    n_threads = host["threads"]
     steps, per = 20000, 4
     shape = inp["radius"] / 10e-9
     q, dq = magpy_amd.core.adjoint_reduced(theta, shape, 400e3, 300.0, 1e-14)
@@ -728,7 +767,7 @@ def adjoint_cpu_baseline(inp, theta, seconds=12.0):
 
     one, m1 = run(1, seconds / 4)
     allc, ma = run(n_threads, seconds)
-    return {"value": allc, "unit": "gradient particle-steps/s", "cores": n_threads,
+    return {"value": allc, "unit": "gradient particle-steps/s", "cores": n_threads, "host": host,
             "kind": "port", "value_1core": one,
             "sample": "%d chunks of %d particles x %d steps of the FP64 oracle's forward-mode "
                       "tangent (4 directions: dq/d(K, alpha, H, r)) on %d threads; 1-core rate "

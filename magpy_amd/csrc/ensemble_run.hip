@@ -1036,7 +1036,9 @@ ClusterLayout cluster_layout(int64_t nmax, int64_t particles)
 // power-of-two blocks from the smallest that holds the largest cluster up to
 // 1024 lanes) on throw-away copies of the state, and keep the fastest.
 // Results are cached per (largest cluster, particles, precision) for the
-// life of the process.
+// life of the process.  Trials run from the group's own first step (a
+// resumed run starts at start_step), and a group too short to time them
+// keeps the untimed layout, uncached.
 ClusterLayout autotune_cluster_layout(const simulation::EnsembleSpec &sp, const Plan &plan,
                                       int64_t nmax, hipStream_t s)
 {
@@ -1046,6 +1048,9 @@ ClusterLayout autotune_cluster_layout(const simulation::EnsembleSpec &sp, const 
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
     const int64_t trial_steps = 64;
+    const int64_t first = std::min(sp.start_step, plan.group_steps[0]);
+    if (plan.group_steps[0] - first < 4 + trial_steps)
+        return cluster_layout(nmax, (int64_t)plan.perm.size());
     hipEvent_t e0, e1;
     MP_HIP(hipEventCreate(&e0));
     MP_HIP(hipEventCreate(&e1));
@@ -1060,9 +1065,9 @@ ClusterLayout autotune_cluster_layout(const simulation::EnsembleSpec &sp, const 
                 g.reset(new GroupRun<float>(sp, plan, 0, false, false, b, lanes));
             else
                 g.reset(new GroupRun<double>(sp, plan, 0, false, false, b, lanes));
-            g->enqueue(4, s);  // warm the code object
+            g->enqueue(g->n_done + 4, s);  // warm the code object
             MP_HIP(hipEventRecord(e0, s));
-            g->enqueue(4 + trial_steps, s);
+            g->enqueue(g->n_done + trial_steps, s);
             MP_HIP(hipEventRecord(e1, s));
             MP_HIP(hipStreamSynchronize(s));
             float ms = 0.f;
