@@ -133,6 +133,22 @@ typedef struct mp_ensemble_out {
  * joblib fan-out of EnsembleModel.simulate (magpy/model.py:202-207). */
 int mp_ensemble_run(const mp_ensemble_spec *spec, mp_ensemble_out *out);
 
+/* mp_ensemble_run without device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code that must
+ * not call hipMalloc, serving loops, harnesses that forbid allocation).  Same
+ * arguments, semantics and results as mp_ensemble_run (the same kernels), but
+ * every device buffer of the run (parameters, state, samples, ensemble
+ * partial sums, field tables) is placed in the library's fixed scratch array
+ * of 64 MiB per GPU (a static device array, held by each GPU the library runs
+ * on until the process ends); launches go to the null stream, no stream or
+ * event is created and the cluster layout is not autotuned (kernel_ms is the
+ * host's clock around the synchronised launches).  A run that needs more
+ * than the scratch returns MP_ERR_ALLOC with the sizes in mp_last_error()
+ * before any kernel runs.  Calls that overlap on one GPU (from several host
+ * threads) share the array and race; each call writes every buffer it reads,
+ * so nothing of an earlier call's contents is used. */
+int mp_ensemble_run_noalloc(const mp_ensemble_spec *spec, mp_ensemble_out *out);
+
 /* One cluster, argument for argument simulation::full_dynamics (physical
  * overload, include/simulation.hpp:74-94) as core.simulate binds it
  * (magpy/core.pyx:72-93); outputs particle-major [n][max_samples] like the

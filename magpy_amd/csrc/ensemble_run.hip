@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <tuple>
@@ -848,6 +850,8 @@ struct mp_session {
     HostCopy host;
     std::vector<std::unique_ptr<GroupBase>> groups;
     StreamGuard sg;
+    // own_stream false: launches go to the null stream and nothing is created
+    explicit mp_session(bool own_stream = true) : sg(own_stream) {}
     bool want_traj = false, want_ens = false;
     int device = 0;
     int cluster_block = 0;
@@ -1085,8 +1089,12 @@ ClusterLayout autotune_cluster_layout(const simulation::EnsembleSpec &sp, const 
     return best;
 }
 
+// With an arena open on this thread (the no-allocation entry point), every
+// device buffer of the session comes from it, launches go to the null
+// stream, no stream or event is created and the layout is not autotuned.
 mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool ens)
 {
+    const bool in_arena = current_arena() != nullptr;
     simulation::validate(sp);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -1094,7 +1102,7 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
     require(sp.device >= 0 && sp.device < ndev, "device ordinal out of range");
     MP_HIP(hipSetDevice(sp.device));
 
-    std::unique_ptr<mp_session> ss(new mp_session());
+    std::unique_ptr<mp_session> ss(new mp_session(!in_arena));
     ss->device = sp.device;
     ss->want_traj = traj;
     ss->want_ens = ens;
@@ -1187,7 +1195,7 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
                                 "one cluster with the Philox stream");
     // pair sums: pick the block size by timing the candidates on long runs
     if (cluster_block != 0 && std::getenv("MAGPY_AMD_CLUSTER_BLOCK") == nullptr &&
-        sp.noise == MP_NOISE_PHILOX && plan.group_steps[0] >= 20000)
+        sp.noise == MP_NOISE_PHILOX && plan.group_steps[0] >= 20000 && !in_arena)
         layout = autotune_cluster_layout(sp, plan, nmax, ss->sg.s);
     cluster_block = layout.block;
     ss->dipolar = cluster_block ? MP_DIPOLAR_PAIRS : 0;
@@ -1207,6 +1215,16 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
 void session_advance(mp_session *ss, int64_t n)
 {
     MP_HIP(hipSetDevice(ss->device));
+    if (ss->sg.e0 == nullptr) {
+        // no events of its own: the host's clock around the synchronised block
+        const auto t0 = std::chrono::steady_clock::now();
+        for (auto &g : ss->groups) g->enqueue(g->n_done + n, ss->sg.s);
+        MP_HIP(hipStreamSynchronize(ss->sg.s));
+        ss->kernel_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count();
+        return;
+    }
     MP_HIP(hipEventRecord(ss->sg.e0, ss->sg.s));
     for (auto &g : ss->groups) g->enqueue(g->n_done + n, ss->sg.s);
     MP_HIP(hipEventRecord(ss->sg.e1, ss->sg.s));
@@ -1325,6 +1343,37 @@ void ensemble_run(const EnsembleSpec &sp, EnsembleResult &out)
 
 }  // namespace simulation
 
+// ============================================ fixed scratch (no allocation)
+// Device memory of mp_ensemble_run_noalloc: one array per GPU, placed when
+// the library's code object loads on that GPU and held until the process
+// ends.  Host code never uses the name `scratch` as an address (on the host
+// it is only a stand-in for the symbol): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The calling thread's current device's copy of `scratch`.
+void *scratch_on(int device)
+{
+    require(device >= 0 && device < kMaxDevices, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device] == nullptr) {
+        void *ptr = nullptr;
+        MP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        require(ptr != nullptr, "no device address for the scratch array");
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -1332,6 +1381,24 @@ int mp_ensemble_run(const mp_ensemble_spec *spec, mp_ensemble_out *out)
 {
     return guarded([&] {
         if (!spec || !out) throw simulation::Error(MP_ERR_ARG, "spec and out are required");
+        simulation::ensemble_run(*spec, *out);
+    });
+}
+
+int mp_ensemble_run_noalloc(const mp_ensemble_spec *spec, mp_ensemble_out *out)
+{
+    return guarded([&] {
+        if (!spec || !out) throw simulation::Error(MP_ERR_ARG, "spec and out are required");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw simulation::Error(MP_ERR_NODEV, "no HIP device visible");
+        require(spec->device >= 0 && spec->device < ndev, "device ordinal out of range");
+        MP_HIP(hipSetDevice(spec->device));
+        Arena arena;
+        arena.base = static_cast<char *>(scratch_on(spec->device));
+        arena.cap = kScratchBytes;
+        arena.who = "mp_ensemble_run_noalloc";
+        ArenaScope scope(&arena);
         simulation::ensemble_run(*spec, *out);
     });
 }

@@ -1,0 +1,163 @@
+"""mp_ensemble_run_noalloc (no device allocation; runs in the library's
+fixed scratch array) against mp_ensemble_run, both through the C ABI.
+
+Two calls in a row, the second with a smaller input, each compared with the
+allocating entry point's result for the same spec: every output array equal
+bit for bit (the two entry points run the same kernels on the same layouts).
+The second call reuses the scratch the first call left behind, so stale
+contents must not leak into it.  A run larger than the scratch is refused
+with MP_ERR_ALLOC and a message naming the scratch, and a valid call after
+it still works.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MU0 = 1.25663706e-6
+SCRATCH_BYTES = 64 << 20
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
+
+
+class Spec(C.Structure):     # mp_ensemble_spec, include/magpy_amd.h
+    _fields_ = [("n_members", C.c_int64), ("member_offsets", _i64p),
+                ("radius", _dp), ("anisotropy", _dp), ("anisotropy_axis", _dp),
+                ("magnetisation_direction", _dp), ("location", _dp),
+                ("magnetisation", _dp), ("damping", _dp), ("temperature", _dp),
+                ("field_shape", _i32p), ("field_amplitude", _dp), ("field_frequency", _dp),
+                ("seed", _i64p), ("time_step", C.c_double), ("end_time", C.c_double),
+                ("max_samples", C.c_int64), ("renorm", C.c_int32), ("interactions", C.c_int32),
+                ("scheme", C.c_int32), ("precision", C.c_int32), ("implicit_tol", C.c_double),
+                ("implicit_max_iter", C.c_int32), ("noise", C.c_int32), ("noise_array", _dp),
+                ("noise_steps", C.c_int64), ("noise_tag", C.c_uint32), ("device", C.c_int32),
+                ("dipolar", C.c_int32), ("start_step", C.c_int64), ("start_state", _dp)]
+
+
+class Out(C.Structure):      # mp_ensemble_out
+    _fields_ = [("time", _dp), ("field", _dp), ("member_time", _dp), ("member_field", _dp),
+                ("mx", _dp), ("my", _dp), ("mz", _dp), ("ens_m", _dp), ("final_state", _dp),
+                ("status", _u32p), ("n_steps", C.c_int64), ("newton_iters", C.c_int64),
+                ("newton_failures", C.c_int64), ("kernel_ms", C.c_double)]
+
+
+def lib():
+    L = C.CDLL(os.path.join(ROOT, "magpy_amd", "libmagpy_amd.so"))
+    for name in ("mp_ensemble_run", "mp_ensemble_run_noalloc"):
+        getattr(L, name).argtypes = [C.POINTER(Spec), C.POINTER(Out)]
+        getattr(L, name).restype = C.c_int
+    L.mp_last_error.restype = C.c_char_p
+    return L
+
+
+def ensemble(sizes, seed, precision, scheme=0, interactions=False, samples=11, steps=600,
+             renorm=False, shape=0):
+    """Arrays of one run: clusters of the given sizes (radii lognormal about
+    10 nm, laid along x 25 nm apart with 3 nm of jitter) in a sine field."""
+    rng = np.random.default_rng(seed)
+    M = len(sizes)
+    P = int(sum(sizes))
+    offs = np.zeros(M + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(sizes)
+    axes = rng.normal(size=(P, 3))
+    axes /= np.linalg.norm(axes, axis=1, keepdims=True)
+    loc = np.zeros((P, 3))
+    for m in range(M):
+        n = sizes[m]
+        loc[offs[m]:offs[m + 1], 0] = 25e-9 * np.arange(n)
+        loc[offs[m]:offs[m + 1], 1] = 3e-9 * rng.normal(size=n)
+    a = dict(offs=offs, radius=10e-9 * np.exp(0.1 * rng.normal(size=P)),
+             K=np.full(P, 23e3), axes=axes, m0=axes.copy(), loc=loc,
+             ms=np.full(M, 4e5), alpha=np.full(M, 0.1), T=np.full(M, 300.0),
+             shape=np.full(M, shape, dtype=np.int32), H=np.full(M, 0.02 / MU0),
+             f=np.full(M, 3e8), seeds=rng.integers(0, 2 ** 31 - 1, size=M).astype(np.int64))
+    dt = 5e-13 if scheme == 1 else 1e-14
+    sp = Spec(n_members=M, time_step=dt, end_time=steps * dt, max_samples=samples,
+              renorm=int(renorm), interactions=int(interactions), scheme=scheme,
+              precision=precision, implicit_tol=1e-9, implicit_max_iter=1000, noise=0,
+              device=0, dipolar=0, start_step=0)
+    for fld, key, t in (("member_offsets", "offs", _i64p), ("radius", "radius", _dp),
+                        ("anisotropy", "K", _dp), ("anisotropy_axis", "axes", _dp),
+                        ("magnetisation_direction", "m0", _dp), ("location", "loc", _dp),
+                        ("magnetisation", "ms", _dp), ("damping", "alpha", _dp),
+                        ("temperature", "T", _dp), ("field_shape", "shape", _i32p),
+                        ("field_amplitude", "H", _dp), ("field_frequency", "f", _dp),
+                        ("seed", "seeds", _i64p)):
+        setattr(sp, fld, a[key].ctypes.data_as(t))
+    return sp, a, M, P
+
+
+def run(fn, sp, M, P, S):
+    o = dict(time=np.full(S, -1.0), field=np.full(S, -1.0), member_time=np.full((M, S), -1.0),
+             member_field=np.full((M, S), -1.0), mx=np.full((P, S), -1.0),
+             my=np.full((P, S), -1.0), mz=np.full((P, S), -1.0), ens_m=np.full((S, 3), -1.0),
+             final_state=np.full((P, 3), -1.0), status=np.full(P, 99, dtype=np.uint32))
+    out = Out()
+    for k, v in o.items():
+        setattr(out, k, v.ctypes.data_as(_u32p if k == "status" else _dp))
+    rc = fn(C.byref(sp), C.byref(out))
+    o["n_steps"] = out.n_steps
+    o["newton_iters"] = out.newton_iters
+    o["newton_failures"] = out.newton_failures
+    return rc, o
+
+
+def same(a, b):
+    for k in a:
+        if isinstance(a[k], np.ndarray):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+        else:
+            assert a[k] == b[k], k
+
+
+CASES = [
+    # (label, sizes, precision, scheme, interactions, samples, steps, renorm)
+    ("fp32 heun 4096 particles", [1] * 4096, 0, 0, False, 21, 2000, False),
+    ("fp64 clusters 2-40", list(np.random.default_rng(3).integers(2, 41, size=40)), 1, 0, True,
+     7, 900, True),
+    ("fp64 implicit 300 particles", [1] * 300, 1, 1, False, 5, 40, False),
+    ("fp32 clusters 2-12", list(np.random.default_rng(4).integers(2, 13, size=25)), 0, 0, True,
+     5, 500, False),
+]
+
+
+def test_noalloc_equals_allocating_entry_point():
+    L = lib()
+    # two calls in a row; each later one smaller than the one before
+    for i, (label, sizes, prec, scheme, inter, S, steps, renorm) in enumerate(CASES):
+        print("CASE noalloc", i, label, flush=True)
+        sp, keep, M, P = ensemble(sizes, 10 + i, prec, scheme, inter, S, steps, renorm)
+        rc_a, a = run(L.mp_ensemble_run, sp, M, P, S)
+        assert rc_a == 0, L.mp_last_error()
+        rc_b, b = run(L.mp_ensemble_run_noalloc, sp, M, P, S)
+        assert rc_b == 0, L.mp_last_error()
+        same(a, b)
+        assert np.all(b["status"] == 0) and b["n_steps"] > 0
+        if scheme == 1:
+            assert b["newton_iters"] > 0 and b["newton_failures"] == 0
+
+
+def test_noalloc_refuses_runs_beyond_the_scratch():
+    L = lib()
+    print("CASE noalloc oversize", flush=True)
+    # 8,192 FP32 particles with 1,001 kept samples: 98 MB of device samples
+    sp, keep, M, P = ensemble([1] * 8192, 5, 0, samples=1001, steps=1000)
+    rc, o = run(L.mp_ensemble_run_noalloc, sp, M, P, 1001)
+    assert rc == 5, rc                      # MP_ERR_ALLOC
+    assert np.all(o["mz"] == -1.0) and np.all(o["status"] == 99)   # nothing written
+    msg = L.mp_last_error().decode()
+    assert "fixed scratch of %d bytes" % SCRATCH_BYTES in msg, msg
+    # a smaller run fits after the refusal and gives the allocating entry
+    # point's result
+    sp, keep, M, P = ensemble([1] * 8192, 6, 0, samples=9, steps=800)
+    rc_a, a = run(L.mp_ensemble_run, sp, M, P, 9)
+    rc_b, b = run(L.mp_ensemble_run_noalloc, sp, M, P, 9)
+    assert rc_a == 0 and rc_b == 0, L.mp_last_error()
+    same(a, b)
