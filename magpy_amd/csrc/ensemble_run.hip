@@ -252,6 +252,7 @@ struct GroupRun : GroupBase {
     int max_iter = 1000;
     double eps = 1e-9;
     bool array_noise = false, want_traj = false, want_ens = false, cluster = false;
+    bool interactions = true;  // clusters: dipolar sums (the implicit scheme may run without)
     int two_wave = 0;          // Heun kernel: 0 one wave, 1/2 two waves (two_wave_mode)
     int lanes = 1;             // lanes per cluster slot (1 or 2)
     int64_t S = 0, s_next = 0, spl = 1;
@@ -284,6 +285,7 @@ struct GroupRun : GroupBase {
         eps = sp.implicit_tol;
         max_iter = sp.implicit_max_iter > 0 ? sp.implicit_max_iter : 1000;
         array_noise = sp.noise == MP_NOISE_ARRAY;
+        interactions = sp.interactions != 0;
         want_traj = traj;
         want_ens = ens;
         spl = steps_per_launch(sp.scheme, sp.precision);
@@ -447,11 +449,37 @@ struct GroupRun : GroupBase {
 
     void launch_cluster(hipStream_t s, const float *htab)
     {
-        if (lanes == 2) launch_cluster_lanes<2>(s, htab);
-        else launch_cluster_lanes<1>(s, htab);
+        if constexpr (sizeof(T) == 8) {
+            if (scheme == MP_SCHEME_IMPLICIT_MIDPOINT) {
+                if (array_noise) launch_implicit_cluster<true>(s);
+                else launch_implicit_cluster<false>(s);
+                return;
+            }
+        }
+        if (array_noise) launch_cluster_lanes<1, true>(s, htab);  // one lane per slot
+        else if (lanes == 2) launch_cluster_lanes<2, false>(s, htab);
+        else launch_cluster_lanes<1, false>(s, htab);
     }
 
-    template <int L>
+    template <bool kArray>
+    void launch_implicit_cluster(hipStream_t s)
+    {
+        mp::ImplicitClusterArgs ia{
+            mp::ClusterArgs<double>{a, d_px.p, d_py.p, d_pz.p, d_w.p, d_cbase.p, d_csize.p}, eps,
+            max_iter, d_it.p, d_fl.p, interactions ? 1 : 0};
+        const size_t shmem = 3 * (size_t)block * sizeof(mp::Vec4<double>);
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, s, ia);
+        };
+        switch (shape_kind) {
+        case mp::kSine: go(mp::implicit_cluster_kernel<mp::kSine, kArray>); break;
+        case mp::kSquare: go(mp::implicit_cluster_kernel<mp::kSquare, kArray>); break;
+        case mp::kConstant: go(mp::implicit_cluster_kernel<mp::kConstant, kArray>); break;
+        default: go(mp::implicit_cluster_kernel<mp::kMixed, kArray>); break;
+        }
+    }
+
+    template <int L, bool kArray>
     void launch_cluster_lanes(hipStream_t s, const float *htab)
     {
         mp::ClusterArgs<T> ca{a, d_px.p, d_py.p, d_pz.p, d_w.p, d_cbase.p, d_csize.p};
@@ -461,26 +489,26 @@ struct GroupRun : GroupBase {
         };
         const bool rn = a.renorm != 0;
         if (htab) {
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true, L>);
-            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false, L>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true, L, kArray>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false, L, kArray>);
             return;
         }
         switch (shape_kind) {
         case mp::kSine:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true, L>);
-            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false, L>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true, L, kArray>);
+            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false, L, kArray>);
             break;
         case mp::kSquare:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true, L>);
-            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false, L>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true, L, kArray>);
+            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false, L, kArray>);
             break;
         case mp::kConstant:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true, L>);
-            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false, L>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true, L, kArray>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false, L, kArray>);
             break;
         default:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true, L>);
-            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false, L>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true, L, kArray>);
+            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false, L, kArray>);
             break;
         }
     }
@@ -959,20 +987,6 @@ void validate(const EnsembleSpec &sp)
         require(sp.magnetisation[m] > 0 && sp.damping[m] >= 0 && sp.temperature[m] >= 0,
                 "magnetisation must be > 0, damping and temperature >= 0");
     }
-    if (sp.interactions) {
-        for (int64_t m = 0; m < sp.n_members; ++m)
-            if (sp.member_offsets[m + 1] - sp.member_offsets[m] > 1) {
-                if (sp.scheme == MP_SCHEME_IMPLICIT_MIDPOINT)
-                    throw Error(MP_ERR_UNSUPPORTED,
-                                "the implicit midpoint scheme with dipolar interactions is not "
-                                "built into this library yet; use the Heun scheme or "
-                                "interactions=0");
-                if (sp.noise == MP_NOISE_ARRAY)
-                    throw Error(MP_ERR_UNSUPPORTED,
-                                "caller-given Wiener increments are not supported for "
-                                "interacting clusters; use the Philox stream");
-            }
-    }
 }
 
 }  // namespace simulation
@@ -987,12 +1001,28 @@ struct ClusterLayout {
 };
 
 // The smallest power-of-two block of at least one wave whose slots hold the
-// largest cluster (0: none up to 1024 lanes).
-int cluster_min_block(int64_t nmax, int lanes)
+// largest cluster (0: none up to max_block lanes).
+int cluster_min_block(int64_t nmax, int lanes, int max_block = 1024)
 {
-    for (int b = mp::kWave; b <= 1024; b *= 2)
+    for (int b = mp::kWave; b <= max_block; b *= 2)
         if (b / lanes >= nmax) return b;
     return 0;
+}
+
+// What the cluster kernels of a run allow: the implicit midpoint kernel and
+// caller-given noise run one lane per slot; the implicit kernel's workgroups
+// hold at most kImplicitClusterBlock lanes (FP64 Newton state per lane).
+struct ClusterLimits {
+    bool one_lane = false;
+    int max_block = 1024;
+};
+
+ClusterLimits cluster_limits(const simulation::EnsembleSpec &sp)
+{
+    ClusterLimits c;
+    c.one_lane = sp.scheme == MP_SCHEME_IMPLICIT_MIDPOINT || sp.noise == MP_NOISE_ARRAY;
+    if (sp.scheme == MP_SCHEME_IMPLICIT_MIDPOINT) c.max_block = mp::kImplicitClusterBlock;
+    return c;
 }
 
 // Two lanes per particle pay while one lane per particle would leave fewer
@@ -1011,10 +1041,12 @@ int env_cluster_lanes()
 // Untimed choice of the layout: MAGPY_AMD_CLUSTER_LANES and
 // MAGPY_AMD_CLUSTER_BLOCK (lanes per workgroup) when set and valid, else two
 // lanes per slot for ensembles of few waves, and the smallest block.
-ClusterLayout cluster_layout(int64_t nmax, int64_t particles)
+ClusterLayout cluster_layout(int64_t nmax, int64_t particles, ClusterLimits lim = {})
 {
     ClusterLayout l;
-    if (const int forced = env_cluster_lanes()) {
+    if (lim.one_lane) {
+        l.lanes = 1;
+    } else if (const int forced = env_cluster_lanes()) {
         l.lanes = forced;
     } else {
         int dev = 0, cus = 0;
@@ -1022,15 +1054,15 @@ ClusterLayout cluster_layout(int64_t nmax, int64_t particles)
         MP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         l.lanes = particles < (int64_t)kSplitBelowWavesPerCU * mp::kWave * cus ? 2 : 1;
     }
-    if (cluster_min_block(nmax, l.lanes) == 0) l.lanes = 1;
+    if (cluster_min_block(nmax, l.lanes, lim.max_block) == 0) l.lanes = 1;
     if (const char *env = std::getenv("MAGPY_AMD_CLUSTER_BLOCK")) {
         const int v = std::atoi(env);
-        if (v / l.lanes >= nmax && v % mp::kWave == 0 && v <= 1024) {
+        if (v / l.lanes >= nmax && v % mp::kWave == 0 && v <= lim.max_block) {
             l.block = v;
             return l;
         }
     }
-    l.block = cluster_min_block(nmax, l.lanes);
+    l.block = cluster_min_block(nmax, l.lanes, lim.max_block);
     return l;
 }
 
@@ -1158,17 +1190,25 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
             require(plan.group_steps[(size_t)g] <= sp.noise_steps,
                     "Exceeded allocated random number array size (noise_steps < steps)");
     // interacting clusters: one block holds whole clusters, so the block must
-    // be at least the largest cluster
+    // be at least the largest cluster.  The implicit scheme couples a
+    // cluster's particles through its Newton norms even without interactions
+    // (one 3N system per cluster, lib/integrators.cpp:576-651), so it takes
+    // the cluster path for any member of several particles.
     ClusterLayout layout;
     int64_t nmax = 1;
-    if (sp.interactions) {
+    if (sp.interactions || sp.scheme == MP_SCHEME_IMPLICIT_MIDPOINT) {
         for (int64_t m = 0; m < sp.n_members; ++m)
             nmax = std::max(nmax, sp.member_offsets[m + 1] - sp.member_offsets[m]);
         if (nmax > 1) {
-            layout = cluster_layout(nmax, (int64_t)plan.perm.size());
+            const ClusterLimits lim = cluster_limits(sp);
+            layout = cluster_layout(nmax, (int64_t)plan.perm.size(), lim);
             if (layout.block == 0)
-                throw simulation::Error(MP_ERR_UNSUPPORTED,
-                                        "clusters of more than 1024 particles are not supported");
+                throw simulation::Error(
+                    MP_ERR_UNSUPPORTED,
+                    sp.scheme == MP_SCHEME_IMPLICIT_MIDPOINT
+                        ? "the implicit midpoint scheme supports clusters of at most " +
+                              std::to_string(lim.max_block) + " particles"
+                        : std::string("clusters of more than 1024 particles are not supported"));
         }
     }
     int cluster_block = layout.block;
@@ -1195,10 +1235,11 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
                                 "one cluster with the Philox stream");
     // pair sums: pick the block size by timing the candidates on long runs
     if (cluster_block != 0 && std::getenv("MAGPY_AMD_CLUSTER_BLOCK") == nullptr &&
-        sp.noise == MP_NOISE_PHILOX && plan.group_steps[0] >= 20000 && !in_arena)
+        sp.noise == MP_NOISE_PHILOX && sp.scheme == MP_SCHEME_HEUN &&
+        plan.group_steps[0] >= 20000 && !in_arena)
         layout = autotune_cluster_layout(sp, plan, nmax, ss->sg.s);
     cluster_block = layout.block;
-    ss->dipolar = cluster_block ? MP_DIPOLAR_PAIRS : 0;
+    ss->dipolar = cluster_block && sp.interactions ? MP_DIPOLAR_PAIRS : 0;
     ss->cluster_block = cluster_block;
     ss->cluster_lanes = cluster_block ? layout.lanes : 0;
     for (int g = 0; g < G; ++g) {

@@ -724,8 +724,10 @@ __device__ inline void dipolar_field(const Vec4<T> *pos, const Vec4<T> *m, int s
 
 // kLanes lanes per slot: the block holds blockDim.x / kLanes slots; with two,
 // the lanes of a slot split its pair sums and repeat the rest of the step, and
-// the even lane alone writes the shared state, samples and results.
-template <typename T, int kShape, bool kTable, bool kRenorm, int kLanes>
+// the even lane alone writes the shared state, samples and results.  With
+// kArrayNoise the Wiener normals are the caller's ([step][slot][3], the
+// reference's RngArray, lib/rng.cpp:67-94) instead of the Philox stream.
+template <typename T, int kShape, bool kTable, bool kRenorm, int kLanes, bool kArrayNoise>
 __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
                                                            const float *__restrict__ htab)
 {
@@ -765,14 +767,24 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
     // the normals of step group g+1 are drawn while group g is integrated
     T zc[12], zn[12];
     const int64_t g_begin = n0 >> 2, g_end = (n1 + 3) >> 2;
-    group_normals(g_begin, pid, a.tag, keys, sig, zc);
+    if constexpr (!kArrayNoise) group_normals(g_begin, pid, a.tag, keys, sig, zc);
     for (int64_t g = g_begin; g < g_end; ++g) {
-        group_normals(g + 1, pid, a.tag, keys, sig, zn);
+        if constexpr (!kArrayNoise) group_normals(g + 1, pid, a.tag, keys, sig, zn);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int64_t n = 4 * g + u;
             if (n < n0 || n >= n1) continue;
-            const T sx = zc[3 * u], sy = zc[3 * u + 1], sz = zc[3 * u + 2];
+            T sx, sy, sz;
+            if constexpr (kArrayNoise) {
+                const double *src = a.noise + ((size_t)n * (size_t)a.n + (size_t)p) * 3;
+                sx = sig * (T)src[0];
+                sy = sig * (T)src[1];
+                sz = sig * (T)src[2];
+            } else {
+                sx = zc[3 * u];
+                sy = zc[3 * u + 1];
+                sz = zc[3 * u + 2];
+            }
             T h1, h2;
             if constexpr (kTable) {
                 h1 = (T)htab[n - tab0];
@@ -826,8 +838,10 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
                 } while (next_rec == n + 1);
             }
         }
+        if constexpr (!kArrayNoise) {
 #pragma unroll
-        for (int i = 0; i < 12; ++i) zc[i] = zn[i];
+            for (int i = 0; i < 12; ++i) zc[i] = zn[i];
+        }
     }
     if (!lead) return;
     a.mx[p] = mx;
@@ -1039,6 +1053,266 @@ __global__ __launch_bounds__(kBlock) void implicit_indep_kernel(ImplicitArgs ia)
         a.mz[p] = x0[2];
         ia.newton_iters[p] += iters;
         ia.newton_fail[p] += fails;
+        uint32_t st = 0;
+        if (!(isfinite(x0[0]) && isfinite(x0[1]) && isfinite(x0[2]))) st |= 1u;
+        if (fails) st |= 2u;
+        a.status[p] |= st;
+    }
+}
+
+// ------------------------------------------------------------------ implicit midpoint, clusters
+// Interacting clusters in FP64, one lane per particle, whole clusters per
+// workgroup (packed as for the Heun pair sums).  The reference takes one
+// Newton step of a 3N x 3N system per cluster step (lib/integrators.cpp:
+// 576-651) whose Jacobian is block diagonal: the drift Jacobian keeps each
+// particle's own block only (llg::multi_drift_quasijacobian,
+// lib/llg.cpp:378-402; the field Jacobian leaves out interactions,
+// lib/simulation.cpp:292-303) and the diffusion Jacobian is per particle
+// (lib/llg.cpp:409-427).  LU with partial pivoting of a block-diagonal
+// matrix never pivots across blocks, and its eliminations and substitutions
+// only subtract exact zeros outside a block, so the 3N solve is the N 3x3
+// solves of its blocks (solve3) value for value.  What couples the
+// particles is F: the effective field at every iterate includes the dipolar
+// sum over the cluster's iterates, exchanged through LDS each iteration; and
+// the tolerance eps |guess|_2 and the step norm |delta|_2 are taken over the
+// whole cluster (lib/optimisation.cpp:98, 142), summed over its members in
+// component order, so the lanes of a cluster iterate together.  A singular
+// block makes the whole system singular: dgesv then leaves its right-hand
+// side -F as the result for every particle of the cluster.  Each particle's
+// drift Jacobian reads the field Jacobian at hj + 3n, n its index in the
+// cluster (lib/llg.cpp:387): a slice of row 0 (and row 1) of the 3N x 3N
+// block-diagonal k e e^T, which for n >= 1 is not the particle's own block.
+// It is kept as written: it steers Newton's path and iteration count, not
+// the root.  Newton counts are kept on each cluster's first lane, so their
+// sum over lanes is the sum over clusters of the reference's counts.  Without
+// interactions the dipolar sums are left out and the cluster still shares
+// its norms, as the reference's 3N system does.
+struct ImplicitClusterArgs {
+    ClusterArgs<double> c;
+    double eps;
+    int max_iter;
+    unsigned long long *newton_iters;   // [slots]
+    unsigned long long *newton_fail;    // [slots]
+    int interactions;                   // 0: no dipolar sums (norms stay per cluster)
+};
+
+constexpr int kImplicitClusterBlock = 256;   // largest workgroup, so largest cluster
+
+template <int kShape, bool kArrayNoise>
+__global__ __launch_bounds__(kImplicitClusterBlock) void implicit_cluster_kernel(
+    ImplicitClusterArgs ia)
+{
+    const ClusterArgs<double> &ca = ia.c;
+    const IndepArgs<double> &a = ca.base;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int slots = blockDim.x;
+    Vec4<double> *pos = reinterpret_cast<Vec4<double> *>(smem_raw);
+    Vec4<double> *mb = pos + slots;    // the states the field is taken from
+    Vec4<double> *red = mb + slots;    // vectors whose cluster norm is taken
+
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x * slots + tid;   // the grid covers all slots exactly
+    const int cn = ca.csize[p];
+    const bool live = cn > 0;
+    const int cb = live ? ca.cbase[p] : tid, ce = live ? cb + cn : tid;
+    const bool leader = live && tid == cb;
+
+    double x0[3] = {a.mx[p], a.my[p], a.mz[p]};
+    const double e[3] = {a.ex[p], a.ey[p], a.ez[p]};
+    const double k = a.k[p], al = a.alpha[p], dt = a.dt[p], amp = a.happ[p];
+    const double sr = a.sr[p], fred = a.fred[p];
+    const int shape = a.shape[p];
+    const double msw = a.msw[p];
+    PhiloxKeys keys;
+    keys.init(a.key0[p], a.key1[p]);
+    const uint32_t pid = a.pid[p];
+    const double ah = sqrt(2 * 1000.0 * fabs(log(dt)));
+    const double sdt = sqrt(dt);
+    pos[tid] = Vec4<double>{ca.px[p], ca.py[p], ca.pz[p], ca.w[p]};
+
+    // hj + 3n of the 3N x 3N field Jacobian J[i][j] = k_{i/3} e[i] e[j] for
+    // i/3 == j/3 (lib/field.cpp:159-174): flat index f = 3n + q, row f / 3N,
+    // column f % 3N.  For every n < N that row is 0, 1 or 2, i.e. particle 0's
+    // block row, nonzero in columns 0-2 only.
+    double hj[9];
+    {
+        const int nn = live ? cn : 1, idx = live ? tid - cb : 0;
+        const int first = blockIdx.x * slots + cb;
+        const double k0 = a.k[first];
+        const double e0[3] = {a.ex[first], a.ey[first], a.ez[first]};
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int f = 3 * idx + q, i = f / (3 * nn), j = f % (3 * nn);
+            hj[q] = (i < 3 && j < 3) ? k0 * e0[i < 3 ? i : 0] * e0[j < 3 ? j : 0] : 0.0;
+        }
+    }
+
+    unsigned long long iters = 0, fails = 0;
+    int64_t s = a.s_next;
+    int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+    double zbuf[4];
+    uint64_t have = UINT64_MAX;
+
+    for (int64_t n = a.n0; n < a.n1; ++n) {
+        double w[3];
+        if (kArrayNoise) {
+            const double *src = a.noise + ((size_t)n * (size_t)a.n + (size_t)p) * 3;
+            w[0] = src[0]; w[1] = src[1]; w[2] = src[2];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const uint64_t j = 3 * (uint64_t)n + d;
+                const uint64_t c = j >> 2;
+                if (c != have) {
+                    const u32x4 r = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), pid, a.tag}, keys);
+                    Num<double>::box_muller(r.x, r.y, zbuf[0], zbuf[1]);
+                    Num<double>::box_muller(r.z, r.w, zbuf[2], zbuf[3]);
+                    have = c;
+                }
+                const int jj = (int)(j & 3);
+                w[d] = jj == 0 ? zbuf[0] : jj == 1 ? zbuf[1] : jj == 2 ? zbuf[2] : zbuf[3];
+            }
+        }
+        const double t = (double)(n + 1) * dt;
+        double dwm[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dwm[d] = fmax(-ah, fmin(ah, w[d])) * sdt;
+
+        // Euler guess from sde(x0, t), the field including the cluster's
+        // dipolar sum at x0 (lib/integrators.cpp:605-614)
+        mb[tid] = Vec4<double>{x0[0], x0[1], x0[2], 0.0};
+        __syncthreads();
+        double h[3], hd[3] = {0.0, 0.0, 0.0}, dr[3], B[9], y[3];
+        if (ia.interactions) dipolar_field<double, 1>(pos, mb, tid, cb, ce, 0, hd[0], hd[1], hd[2]);
+        {
+            const double dot = (x0[0] * e[0] + x0[1] * e[1] + x0[2] * e[2]) * k;
+            h[0] = dot * e[0];
+            h[1] = dot * e[1];
+            h[2] = dot * e[2] + Applied<double, kShape>::at(shape, amp, fred, t);
+            if (ia.interactions) {
+                h[0] = h[0] + hd[0];
+                h[1] = h[1] + hd[1];
+                h[2] = h[2] + hd[2];
+            }
+        }
+        drift3(x0, al, h, dr);
+        diffusion3(x0, sr, al, B);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double g = dr[i] * dt;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) g += B[i * 3 + j] * dwm[j];
+            y[i] = (g + x0[i]) / 2;
+        }
+        red[tid] = Vec4<double>{y[0], y[1], y[2], 0.0};
+        __syncthreads();   // every lane's guess field is taken: mb may change
+        double ss = 0.0;
+        for (int j = cb; j < ce; ++j) {
+            const Vec4<double> v = red[j];
+            ss += v.x * v.x;
+            ss += v.y * v.y;
+            ss += v.z * v.z;
+        }
+        const double tol = ia.eps * sqrt(ss);
+        mb[tid] = Vec4<double>{y[0], y[1], y[2], 0.0};
+        const double happ_mid = Applied<double, kShape>::at(shape, amp, fred, t + dt / 2);
+        double err = 2 * tol;
+        int iter = ia.max_iter;
+        bool singular = false;
+        bool go = live && (err > tol) && (iter-- > 0);
+        while (__syncthreads_or(go)) {
+            const double xt[3] = {y[0], y[1], y[2]};
+            double dlt[3] = {0.0, 0.0, 0.0}, nf[3] = {0.0, 0.0, 0.0};
+            bool ok = true;
+            if (go) {
+                const double dot = (xt[0] * e[0] + xt[1] * e[1] + xt[2] * e[2]) * k;
+                h[0] = dot * e[0];
+                h[1] = dot * e[1];
+                h[2] = dot * e[2] + happ_mid;
+                if (ia.interactions) {
+                    dipolar_field<double, 1>(pos, mb, tid, cb, ce, 0, hd[0], hd[1], hd[2]);
+                    h[0] = h[0] + hd[0];
+                    h[1] = h[1] + hd[1];
+                    h[2] = h[2] + hd[2];
+                }
+                drift3(xt, al, h, dr);
+                diffusion3(xt, sr, al, B);
+                double J[9], jb[27], F[3];
+                drift_jac3(xt, al, h, hj, J);
+                diffusion_jac3(xt, sr, al, jb);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    F[i] = xt[i] - 0.5 * dr[i] * dt - x0[i];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) F[i] -= 0.5 * B[i * 3 + j] * dwm[j];
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        double v = (double)(i == j) - 0.5 * J[i * 3 + j];
+#pragma unroll
+                        for (int kk = 0; kk < 3; ++kk) v -= 0.5 * jb[i * 9 + kk * 3 + j] * dwm[kk];
+                        J[i * 3 + j] = v;
+                    }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    dlt[i] = -F[i];
+                    nf[i] = -F[i];
+                }
+                ++iters;
+                ok = solve3(J, dlt);
+            }
+            __syncthreads();   // the iterates' field and the last norm are read
+            if (go) red[tid] = Vec4<double>{dlt[0], dlt[1], dlt[2], ok ? 0.0 : 1.0};
+            __syncthreads();
+            if (go) {
+                double s2 = 0.0;
+                bool sing = false;
+                for (int j = cb; j < ce; ++j) {
+                    const Vec4<double> v = red[j];
+                    s2 += v.x * v.x;
+                    s2 += v.y * v.y;
+                    s2 += v.z * v.z;
+                    sing = sing || v.w != 0.0;
+                }
+                if (sing) {   // LAPACK_ERR: x_root left at -F
+                    y[0] = nf[0]; y[1] = nf[1]; y[2] = nf[2];
+                    singular = true;
+                    go = false;
+                } else {
+                    err = sqrt(s2);
+                    y[0] = dlt[0] + xt[0];
+                    y[1] = dlt[1] + xt[1];
+                    y[2] = dlt[2] + xt[2];
+                    mb[tid] = Vec4<double>{y[0], y[1], y[2], 0.0};
+                    go = (err > tol) && (iter-- > 0);
+                }
+            }
+        }
+        if (live && (singular || iter == -1)) ++fails;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) x0[i] = 2 * y[i] - x0[i];
+        if (a.renorm && live) {
+            const double nrm = sqrt(x0[0] * x0[0] + x0[1] * x0[1] + x0[2] * x0[2]);
+            x0[0] = x0[0] / nrm; x0[1] = x0[1] / nrm; x0[2] = x0[2] / nrm;
+        }
+        if (n + 1 == next_rec) {
+            do {
+                record_sample(a, s, p, live, x0[0], x0[1], x0[2], msw, grid_wave());
+                ++s;
+                next_rec = s < a.S ? a.sched[s] : INT64_MAX;
+            } while (next_rec == n + 1);
+        }
+    }
+    if (live) {
+        a.mx[p] = x0[0];
+        a.my[p] = x0[1];
+        a.mz[p] = x0[2];
+        if (leader) {
+            ia.newton_iters[p] += iters;
+            ia.newton_fail[p] += fails;
+        }
         uint32_t st = 0;
         if (!(isfinite(x0[0]) && isfinite(x0[1]) && isfinite(x0[2]))) st |= 1u;
         if (fails) st |= 2u;

@@ -82,6 +82,22 @@ def test_full_dynamics_ctypes_matches_binding_and_oracle():
     assert np.max(np.abs(o_free["z"] - g["z"])) > 1e-6 * 4e5
 
 
+def test_full_dynamics_ctypes_implicit_dimer():
+    # the reference's default path: implicit midpoint with interactions
+    print("CASE abi_full_dynamics_implicit", flush=True)
+    lib, P = bind()
+    r, K, axes, m0, loc = dimer()
+    dt, S, seed = 5e-13, 11, 77
+    end = 300 * dt
+    rc, g = full_dynamics(lib, P, r, K, axes, m0, loc, True, S, end, dt, seed)
+    assert rc == 0, lib.mp_last_error()
+    o = O.full_dynamics(r, K, axes, m0, loc, 4e5, 0.1, 300.0, False, True, True, 1e-9, dt, end,
+                        S, seed, field_shape='sine', field_amplitude=0.02 / MU0,
+                        field_frequency=3e9, rng='philox')
+    for comp in "xyz":
+        np.testing.assert_allclose(g[comp], o[comp], rtol=0, atol=1e-10 * 4e5)
+
+
 def test_full_dynamics_ctypes_errors():
     print("CASE abi_full_dynamics_errors", flush=True)
     lib, P = bind()
