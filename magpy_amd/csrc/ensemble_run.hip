@@ -461,6 +461,17 @@ struct GroupRun : GroupBase {
         else launch_cluster_lanes<1, false>(s, htab);
     }
 
+    // Partners whose geometry the two-lane FP32 kernel caches per lane (in
+    // workgroups of at most 256 lanes): 16, or 0 with MAGPY_AMD_CLUSTER_CACHE=0.
+    // Results do not depend on it.
+    int cluster_cache() const
+    {
+        if (sizeof(T) != 4 || lanes != 2 || block > 256 || array_noise) return 0;
+        if (const char *env = std::getenv("MAGPY_AMD_CLUSTER_CACHE"))
+            if (std::atoi(env) == 0) return 0;
+        return 16;
+    }
+
     template <bool kArray>
     void launch_implicit_cluster(hipStream_t s)
     {
@@ -482,6 +493,18 @@ struct GroupRun : GroupBase {
     template <int L, bool kArray>
     void launch_cluster_lanes(hipStream_t s, const float *htab)
     {
+        if constexpr (L == 2 && !kArray && sizeof(T) == 4) {
+            if (cluster_cache() == 16) {
+                launch_cluster_variant<L, kArray, 16>(s, htab);
+                return;
+            }
+        }
+        launch_cluster_variant<L, kArray, 0>(s, htab);
+    }
+
+    template <int L, bool kArray, int KC>
+    void launch_cluster_variant(hipStream_t s, const float *htab)
+    {
         mp::ClusterArgs<T> ca{a, d_px.p, d_py.p, d_pz.p, d_w.p, d_cbase.p, d_csize.p};
         const size_t shmem = 3 * (size_t)(block / L) * sizeof(mp::Vec4<T>);
         auto go = [&](auto kernel) {
@@ -489,26 +512,26 @@ struct GroupRun : GroupBase {
         };
         const bool rn = a.renorm != 0;
         if (htab) {
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true, L, kArray>);
-            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false, L, kArray>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true, L, kArray, KC>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false, L, kArray, KC>);
             return;
         }
         switch (shape_kind) {
         case mp::kSine:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true, L, kArray>);
-            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false, L, kArray>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true, L, kArray, KC>);
+            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false, L, kArray, KC>);
             break;
         case mp::kSquare:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true, L, kArray>);
-            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false, L, kArray>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true, L, kArray, KC>);
+            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false, L, kArray, KC>);
             break;
         case mp::kConstant:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true, L, kArray>);
-            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false, L, kArray>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true, L, kArray, KC>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false, L, kArray, KC>);
             break;
         default:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true, L, kArray>);
-            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false, L, kArray>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true, L, kArray, KC>);
+            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false, L, kArray, KC>);
             break;
         }
     }

@@ -722,14 +722,100 @@ __device__ inline void dipolar_field(const Vec4<T> *pos, const Vec4<T> *m, int s
     }
 }
 
+// The pair geometry of a lane's first K partners with two lanes per particle
+// (j = b + half + 2k): positions do not move, so d = p_j - p_i, r^-2 and the
+// weight w_j r^-3 are computed once per launch, by the same expressions as
+// dipolar_term (the self pair and partners past the cluster's end get weight
+// 0), and each stage evaluates only the state-dependent part of those terms.
+template <typename T, int K>
+struct PairCache {
+    T dx[K], dy[K], dz[K], c[K], s[K];
+};
+
+template <typename T, int K>
+__device__ inline void pair_cache_init(const Vec4<T> *pos, int self, int jb, int e,
+                                       PairCache<T, K> &pc)
+{
+#pragma clang fp contract(off)
+    const Vec4<T> pi = pos[self];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int j = jb + 2 * k;
+        const bool in = j < e;
+        const Vec4<T> pj = pos[in ? j : self];
+        const bool skip = !in || j == self;
+        const T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const T r2 = skip ? T(1) : fmad(dx, dx, fmad(dy, dy, dz * dz));
+        const T rinv = __builtin_amdgcn_rsqf(r2);
+        const T rinv2 = rinv * rinv;
+        pc.dx[k] = skip ? T(0) : dx;
+        pc.dy[k] = skip ? T(0) : dy;
+        pc.dz[k] = skip ? T(0) : dz;
+        pc.s[k] = rinv2;
+        pc.c[k] = skip ? T(0) : pj.w * rinv2 * rinv;
+    }
+}
+
+// Two lanes per particle, the first K partners' geometry cached: the same
+// terms, in the same order, as dipolar_field<T, 2>.
+template <typename T, int K>
+__device__ inline void dipolar_field_cached(const Vec4<T> *pos, const Vec4<T> *m, int self, int b,
+                                            int e, int half, const PairCache<T, K> &pc, T &hx,
+                                            T &hy, T &hz)
+{
+#pragma clang fp contract(off)
+    static_assert(K % 4 == 0, "cached partners come in chunks of four");
+    T x = T(0), y = T(0), z = T(0);
+    const int jb = b + half;
+#pragma unroll
+    for (int c4 = 0; c4 < K / 4; ++c4) {
+        if (jb + 8 * c4 < e) {
+            Vec4<T> mj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = jb + 2 * (4 * c4 + u);
+                mj[u] = m[j < e ? j : self];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = 4 * c4 + u;
+                const T t = T(3) * fmad(mj[u].x, pc.dx[k], fmad(mj[u].y, pc.dy[k], mj[u].z * pc.dz[k])) *
+                            pc.s[k];
+                x = fmad(pc.c[k], fmad(t, pc.dx[k], -mj[u].x), x);
+                y = fmad(pc.c[k], fmad(t, pc.dy[k], -mj[u].y), y);
+                z = fmad(pc.c[k], fmad(t, pc.dz[k], -mj[u].z), z);
+            }
+        }
+    }
+    const Vec4<T> pi = pos[self];
+    int j = jb + 2 * K;
+    for (; j + 6 < e; j += 8) {
+        Vec4<T> pj[4], mj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pj[u] = pos[j + 2 * u];
+            mj[u] = m[j + 2 * u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) dipolar_term(pi, pj[u], mj[u], j + 2 * u == self, x, y, z);
+    }
+    for (; j < e; j += 2) dipolar_term(pi, pos[j], m[j], j == self, x, y, z);
+    hx = x + __shfl_xor(x, 1);
+    hy = y + __shfl_xor(y, 1);
+    hz = z + __shfl_xor(z, 1);
+}
+
 // kLanes lanes per slot: the block holds blockDim.x / kLanes slots; with two,
 // the lanes of a slot split its pair sums and repeat the rest of the step, and
 // the even lane alone writes the shared state, samples and results.  With
 // kArrayNoise the Wiener normals are the caller's ([step][slot][3], the
 // reference's RngArray, lib/rng.cpp:67-94) instead of the Philox stream.
-template <typename T, int kShape, bool kTable, bool kRenorm, int kLanes, bool kArrayNoise>
-__global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
-                                                           const float *__restrict__ htab)
+// kCache (two lanes, FP32) caches the first kCache partners' geometry per
+// lane (PairCache): workgroups of at most 256 lanes then, for the registers.
+template <typename T, int kShape, bool kTable, bool kRenorm, int kLanes, bool kArrayNoise,
+          int kCache = 0>
+__global__ __launch_bounds__(kCache > 0 ? 256 : 1024) void heun_cluster_kernel(
+    ClusterArgs<T> ca, const float *__restrict__ htab)
 {
 #pragma clang fp contract(off)
     const IndepArgs<T> &a = ca.base;
@@ -760,6 +846,12 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
     const int64_t n0 = a.n0, n1 = a.n1;
     const int64_t tab0 = 4 * (n0 >> 2);
     if (lead) pos[tid] = Vec4<T>{ca.px[p], ca.py[p], ca.pz[p], ca.w[p]};
+    static_assert(kCache == 0 || (kLanes == 2 && sizeof(T) == 4), "cache: two lanes, FP32");
+    PairCache<T, (kCache > 0 ? kCache : 4)> pc;
+    if constexpr (kCache > 0) {
+        __syncthreads();
+        pair_cache_init(pos, tid, cb + half, ce, pc);
+    }
 
     int64_t s = a.s_next;
     int64_t next_rec = s < a.S ? a.sched[s] : INT64_MAX;
@@ -802,7 +894,10 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
             if (lead) mb0[tid] = Vec4<T>{mx, my, mz, T(0)};
             __syncthreads();
             T hx, hy, hz;
-            dipolar_field<T, kLanes>(pos, mb0, tid, cb, ce, half, hx, hy, hz);
+            if constexpr (kCache > 0)
+                dipolar_field_cached(pos, mb0, tid, cb, ce, half, pc, hx, hy, hz);
+            else
+                dipolar_field<T, kLanes>(pos, mb0, tid, cb, ce, half, hx, hy, hz);
             T d = fmad(mx, kex, fmad(my, key, mz * kez));
             T vx = fmad(d, edx, fmad(hx, dt, sx)), vy = fmad(d, edy, fmad(hy, dt, sy)),
               vz = fmad(d, edz, fmad(hz + h1, dt, sz));
@@ -812,7 +907,10 @@ __global__ __launch_bounds__(1024) void heun_cluster_kernel(ClusterArgs<T> ca,
             // stage 2
             if (lead) mb1[tid] = Vec4<T>{tx, ty, tz, T(0)};
             __syncthreads();
-            dipolar_field<T, kLanes>(pos, mb1, tid, cb, ce, half, hx, hy, hz);
+            if constexpr (kCache > 0)
+                dipolar_field_cached(pos, mb1, tid, cb, ce, half, pc, hx, hy, hz);
+            else
+                dipolar_field<T, kLanes>(pos, mb1, tid, cb, ce, half, hx, hy, hz);
             d = fmad(tx, kex, fmad(ty, key, tz * kez));
             vx = fmad(d, edx, fmad(hx, dt, sx));
             vy = fmad(d, edy, fmad(hy, dt, sy));
