@@ -608,6 +608,167 @@ def grad_check(args):
         print(json.dumps(line), flush=True)
 
 
+def grad_check_scale(args):
+    """Config 4's accuracy at its own scale: 16,384 AC-field particles, 2
+    cycles at 300 kHz (6.67e8 steps), 4,002 samples (with 4,001, every third
+    sample time is an exact multiple of the step, where the ensemble path's
+    schedule, taken in each member's reduced units, can move by one step
+    under a perturbation of K or alpha; 4,002 puts no sample on a step).
+
+    phase 'adjoint' (--precision fp64 or fp32; --grad-check-groups G and
+    --grad-check-group g run the particles g*P/G .. (g+1)*P/G - 1 only, so
+    an FP64 run can be split over sittings): the pathwise gradient of SAR and
+    final <m_z> and each particle's part of the SAR gradient, all scaled to
+    the whole ensemble (SAR and <m_z> are means over particles, so the
+    groups' parts add up), saved to --grad-check-out (npz).
+
+    phase 'fd' (--grad-check-in: the FP64 group files, comma separated;
+    --grad-check-in32: the FP32 file): the particles carrying the largest
+    parts (the 8 largest per component) and 32 drawn at random are run again
+    as independent members at theta +- h theta_i for h in (1e-6, 1e-7,
+    1e-8), with their own Wiener streams (common random numbers), in FP64
+    through the ensemble path; each particle's SAR contribution
+    -mu0 f trapz(H, Ms m_z / P) over the final cycle gives central
+    differences.  A particle's converged difference is the one at the
+    smaller step of the consecutive pair of steps that agree best (chosen
+    from the differences alone); it is compared with the adjoint's part for
+    that particle (plus the explicit SAR_i / H for H)."""
+    import magpy_amd
+    core = magpy_amd.core
+    P, f, dt, S = 16384, 3e5, 1e-14, 4002
+    if args.grad_check_quick:     # the same check over 2 cycles of a 30 MHz field
+        f = 3e7
+    end = 2.0 / f
+    inp = ac_field_inputs(P)
+    shape = inp["radius"] / 10e-9
+    ax = inp["anisotropy_axis"]
+    theta = np.array([23e3, 0.1, 0.02 / MU0, 10e-9])
+    k, n_steps = core.sample_schedule(dt, end, S)
+    ts = np.arange(S) * (end / (S - 1))
+    print("device:", core.device_info(0), flush=True)
+    if args.grad_check_scale == "adjoint":
+        prec = args.precision or "fp64"
+        G, g = max(1, args.grad_check_groups), args.grad_check_group
+        part = np.arange(g * P // G, (g + 1) * P // G)
+        scale = len(part) / P
+        os.environ["MAGPY_AMD_ADJOINT_STEPS_PER_LAUNCH"] = str(1 << 20)
+        run = core.Adjoint(shape[part], ax[part], ax[part], inp["seeds"][part], 400e3, 300.0, f,
+                           dt, n_steps, k, 1000, prec)
+        r = gradient_run(run, theta, ts, f, "scale %s group %d/%d" % (prec, g, G))
+        np.savez(args.grad_check_out, particles=part,
+                 particle_grad_sar=r["particle_grad_sar"] * scale,
+                 grad_sar=r["grad_sar"] * scale, grad_final_mz=r["grad_final_mz"] * scale,
+                 sar=r["sar"] * scale, final_mz=r["final_mz"] * scale, precision=prec)
+        print(json.dumps({"grad_check_scale": "adjoint", "precision": prec, "particles": P,
+                          "group": [g, G], "group_particles": int(len(part)),
+                          "steps": int(n_steps), "samples": S,
+                          "sar_part_W_per_m3": r["sar"] * scale,
+                          "grad_sar_part": (r["grad_sar"] * scale).tolist(),
+                          "grad_final_mz_part": (r["grad_final_mz"] * scale).tolist(),
+                          "wall_s": r["total_s"], "status": r["status"],
+                          "adjoint_info": run.info()}), flush=True)
+        return
+    groups = [np.load(fn) for fn in args.grad_check_in.split(",")]
+    pg = np.zeros((P, 4))
+    for gr in groups:
+        pg[gr["particles"]] = gr["particle_grad_sar"]
+    assert sum(len(gr["particles"]) for gr in groups) == P, "FP64 groups must cover every particle"
+    grad64 = sum(gr["grad_sar"] for gr in groups)
+    gfin64 = sum(gr["grad_final_mz"] for gr in groups)
+    sar64 = float(sum(gr["sar"] for gr in groups))
+    a32 = np.load(args.grad_check_in32) if args.grad_check_in32 else None
+    top = np.unique(np.concatenate([np.argsort(np.abs(pg[:, c]))[-8:] for c in range(4)]))
+    rng = np.random.default_rng(7)
+    rand = np.sort(rng.choice(np.setdiff1d(np.arange(P), top), size=32, replace=False))
+    sel = np.concatenate([top, rand])
+    steps = (1e-6, 1e-7, 1e-8)
+    variants = [(None, 0.0, 0)] + [(c, h, sg) for c in range(4) for h in steps for sg in (1, -1)]
+    M = len(sel) * len(variants)
+    radius, kk, alpha, hamp, seeds, axes = [], [], [], [], [], []
+    for v in variants:
+        th = theta.copy()
+        if v[0] is not None:
+            th[v[0]] += v[2] * v[1] * theta[v[0]]
+        radius.append(shape[sel] * th[3])
+        kk.append(np.full(len(sel), th[0]))
+        alpha.append(np.full(len(sel), th[1]))
+        hamp.append(np.full(len(sel), th[2]))
+        seeds.append(inp["seeds"][sel])
+        axes.append(ax[sel])
+    axes = np.concatenate(axes)
+    t0 = time.perf_counter()
+    ses = core.Session(member_offsets=np.arange(M + 1, dtype=np.int64),
+                       radius=np.concatenate(radius), anisotropy=np.concatenate(kk),
+                       anisotropy_axis=axes, magnetisation_direction=axes,
+                       location=np.zeros((M, 3)), magnetisation=400e3,
+                       damping=np.concatenate(alpha), temperature=300.0, field_shape="sine",
+                       field_amplitude=np.concatenate(hamp), field_frequency=f,
+                       seeds=np.concatenate(seeds), time_step=dt, end_time=end, max_samples=S,
+                       interactions=False, precision="fp64", trajectories=True,
+                       ensemble_sum=False)
+    done, prev = 0, -1
+    while done < n_steps and done != prev:
+        prev = done
+        done = ses.advance(1 << 25)
+        print(json.dumps({"pass": "fd fp64", "progress_steps": int(done), "of": int(n_steps),
+                          "elapsed_s": round(time.perf_counter() - t0, 1)}), flush=True)
+    out = ses.finish()
+    mask = ts >= ts[-1] - 1.0 / f
+    npart = len(sel)
+    sar_i = np.zeros((len(variants), npart))
+    for vi, v in enumerate(variants):
+        h_amp = theta[2] * (1 + (v[2] * v[1] if v[0] == 2 else 0.0))
+        field = h_amp * np.sin(2 * np.pi * f * ts)
+        mz = out["z"][vi * npart:(vi + 1) * npart]          # Ms m_z, A/m
+        for j in range(npart):
+            sar_i[vi, j] = f * -MU0 * np.trapezoid(field[mask], mz[j][mask] / P)
+    fd = np.zeros((len(steps), npart, 4))
+    for c in range(4):
+        for hi, h in enumerate(steps):
+            ip = variants.index((c, h, 1))
+            im = variants.index((c, h, -1))
+            fd[hi, :, c] = (sar_i[ip] - sar_i[im]) / (2 * h * theta[c])
+    # converged difference: the smaller step of the best-agreeing consecutive pair
+    pair = np.abs(fd[1:] - fd[:-1]) / np.maximum(np.abs(fd[1:]), 1e-300)   # [2, n, 4]
+    pick = np.argmin(pair, axis=0) + 1
+    fdc = np.take_along_axis(fd, pick[None], axis=0)[0]
+    conv = np.take_along_axis(pair, (pick - 1)[None], axis=0)[0]
+
+    def rel(adj):
+        a = adj.copy()
+        a[:, 2] += sar_i[0] / theta[2]          # SAR_i is linear in the field amplitude
+        return a, np.abs(a - fdc) / np.maximum(np.abs(fdc), 1e-300)
+
+    adj, err = rel(pg[sel])
+    nt = len(top)
+    scale_abs = np.abs(pg).sum(axis=0)
+    line = {"grad_check_scale": "fd", "particles": P, "steps": int(n_steps), "samples": S,
+            "field_frequency": f, "checked_particles": int(npart), "top": top.tolist(),
+            "random": rand.tolist(), "fd_steps": list(steps), "wall_s": time.perf_counter() - t0,
+            "sar_fp64_W_per_m3": sar64, "grad_sar_fp64": grad64.tolist(),
+            "grad_final_mz_fp64": gfin64.tolist(),
+            "share_of_abs_grad_in_top": (np.abs(pg[top]).sum(axis=0) / scale_abs).tolist(),
+            "fd_self_agreement_max_top": conv[:nt].max(axis=0).tolist(),
+            "fd_self_agreement_max_random": conv[nt:].max(axis=0).tolist(),
+            "max_rel_err_fp64_top": err[:nt].max(axis=0).tolist(),
+            "max_rel_err_fp64_random": err[nt:].max(axis=0).tolist(),
+            "median_rel_err_fp64_random": np.median(err[nt:], axis=0).tolist(),
+            "checked_sum_rel_err_fp64": (np.abs(adj.sum(0) - fdc.sum(0)) /
+                                         np.abs(fdc.sum(0))).tolist(),
+            "status": int(np.max(out["status"]))}
+    if a32 is not None:
+        adj32, err32 = rel(a32["particle_grad_sar"][sel])
+        g32 = a32["grad_sar"]
+        line["grad_sar_fp32"] = g32.tolist()
+        line["grad_sar_fp32_vs_fp64"] = (np.abs(g32 - grad64) / np.abs(grad64)).tolist()
+        line["max_rel_err_fp32_top"] = err32[:nt].max(axis=0).tolist()
+        line["max_rel_err_fp32_random"] = err32[nt:].max(axis=0).tolist()
+        line["median_rel_err_fp32_random"] = np.median(err32[nt:], axis=0).tolist()
+    np.savez(args.grad_check_out or os.path.join(ROOT, "gpurun_out", "grad_scale_fd.npz"),
+             sel=sel, fd=fd, adj=adj, sar_i=sar_i)
+    print(json.dumps(line), flush=True)
+
+
 DOM_RUN = dict(time_step=1e-12, end_time=5 / 3e5, max_samples=11, field_shape="sine",
                field_amplitude=0.02 / MU0, field_frequency=3e5)
 
@@ -822,6 +983,14 @@ def main():
                     help="with --grad-check: only the full 300 kHz gradient run")
     ap.add_argument("--grad-check-quick", action="store_true",
                     help="with --grad-check: only the finite-difference check")
+    ap.add_argument("--grad-check-scale", default="", choices=["", "adjoint", "fd"],
+                    help="the differentiation workload's gradient checked at its own scale")
+    ap.add_argument("--grad-check-out", default="", help="npz written by --grad-check-scale")
+    ap.add_argument("--grad-check-in", default="", help="FP64 adjoint npz for the fd phase")
+    ap.add_argument("--grad-check-in32", default="", help="FP32 adjoint npz for the fd phase")
+    ap.add_argument("--grad-check-groups", type=int, default=1,
+                    help="particle groups of the adjoint phase (one per sitting)")
+    ap.add_argument("--grad-check-group", type=int, default=0, help="the group this run takes")
     ap.add_argument("--noise-tag", type=int, default=0,
                     help="Philox stream tag of the --sar-run (another thermal history)")
     args = ap.parse_args()
@@ -833,6 +1002,9 @@ def main():
         return
     if args.replica_accuracy:
         replica_accuracy(args)
+        return
+    if args.grad_check_scale:
+        grad_check_scale(args)
         return
     if args.grad_check:
         grad_check(args)

@@ -676,8 +676,9 @@ void launch_product(int mode, const float *a32, const float *b32, const __bf16 *
         hipLaunchKernelGGL(mp::gemm_f32_kernel, grid, dim3(256), 0, s, a32, b32, c, ld, ld);
     } else {
         const int nrb = rows / mp::kBM, ncb = ld / mp::kBN;
-        hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel, dim3((unsigned)(nrb * ncb)), dim3(512),
-                           0, s, a32, b_hi, b_lo, c, ld, ld, nrb, ncb);
+        hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel<0>, dim3((unsigned)(nrb * ncb)), dim3(512),
+                           0, s, a32, b_hi, b_lo, c, ld, ld, nrb, ncb, mp::ReplicaArgs{},
+                           (int64_t)0, 0.f, (int64_t)0, 0);
     }
 }
 
@@ -776,7 +777,8 @@ struct ReplicaRun : GroupBase {
         d_sched.upload(sched);
         const int threads = reps * n;
         const int grid = (threads + 255) / 256;
-        n_waves = grid * 4;
+        // the fused BF16x3 stages write one ensemble partial per product wave
+        n_waves = mode == MP_DIPOLAR_GEMM_BF16X3 ? (rows / mp::kBM) * (ld / mp::kBN) * 8 : grid * 4;
         if (want_traj) d_samples.alloc((size_t)S * 3 * (size_t)threads);
         if (want_ens) {
             d_partial.alloc((size_t)S * 3 * (size_t)n_waves);
@@ -815,14 +817,28 @@ struct ReplicaRun : GroupBase {
             const double t1 = (double)(st + 1) * dt;
             const float h1 = (float)applied_host(shape, happ, fapp, t1);
             const float h2 = (float)applied_host(shape, happ, fapp, t1 + dt);
-            launch_product(mode, d_m.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows, ld, s);
-            hipLaunchKernelGGL(mp::replica_stage1_kernel, grid, dim3(256), 0, s, a, st, h1);
-            launch_product(mode, d_mt.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows, ld, s);
             while (s_next < S && sched[(size_t)s_next] < st + 1) ++s_next;
             int n_rec = 0;
             while (s_next + n_rec < S && sched[(size_t)(s_next + n_rec)] == st + 1) ++n_rec;
-            hipLaunchKernelGGL(mp::replica_stage2_kernel, grid, dim3(256), 0, s, a, st, h2,
-                               s_next, n_rec);
+            if (mode == MP_DIPOLAR_GEMM_BF16X3) {
+                // the stages run in the products' epilogues
+                require(rows % mp::kBM == 0 && ld % mp::kBN == 0 && ld % (6 * mp::kBK) == 0,
+                        "replica product: rows or ld do not fit the tiles");
+                const int nrb = rows / mp::kBM, ncb = ld / mp::kBN;
+                const dim3 pg((unsigned)(nrb * ncb)), pb(512);
+                hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel<1>, pg, pb, 0, s, d_m.p, d_bhi.p,
+                                   d_blo.p, (float *)nullptr, ld, ld, nrb, ncb, a, st, h1,
+                                   (int64_t)0, 0);
+                hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel<2>, pg, pb, 0, s, d_mt.p, d_bhi.p,
+                                   d_blo.p, (float *)nullptr, ld, ld, nrb, ncb, a, st, h2, s_next,
+                                   n_rec);
+            } else {
+                launch_product(mode, d_m.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows, ld, s);
+                hipLaunchKernelGGL(mp::replica_stage1_kernel, grid, dim3(256), 0, s, a, st, h1);
+                launch_product(mode, d_mt.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows, ld, s);
+                hipLaunchKernelGGL(mp::replica_stage2_kernel, grid, dim3(256), 0, s, a, st, h2,
+                                   s_next, n_rec);
+            }
             MP_HIP(hipGetLastError());
             ++launches;
             n_done = st + 1;

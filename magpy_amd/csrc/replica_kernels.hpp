@@ -61,6 +61,124 @@ __device__ inline void store_tile32(float *C, int ldc, int r0, int c0, const f32
     }
 }
 
+// ------------------------------------------------------------------ Heun stages
+// Per-replica Heun stages around the product.  Thread = (replica r, particle
+// p), p fastest; state rows are replica-major with leading dimension ld
+// (>= 3N, padded).  The applied field of the two stage times is uniform and
+// passed in (h1, h2).
+struct ReplicaArgs {
+    int n;                  // particles per replica
+    int reps;               // replicas
+    int ld;                 // row length 3 Np (Np = n padded to a multiple of 128)
+    int n_waves;            // waves in the grid (ensemble partial stride)
+    const float *kex, *key, *kez, *edx, *edy, *edz;  // per particle: k e, e dt
+    const float *sig;       // per particle: sr sqrt(dt)
+    float dt, alpha;
+    const uint32_t *key0, *key1;   // per replica Philox key (member seed)
+    const double *msw;             // per replica Ms
+    uint32_t tag;
+    int renorm;
+    float *m, *mt, *h;             // [reps][ld]: state, predictor, field
+    uint32_t *status;              // [reps * n]
+    // sampling
+    const int64_t *sched;
+    int64_t S;
+    float *samples;                // [S][3][reps * n] or null
+    double *ens_partial;           // [S][3][n_waves] or null
+};
+
+// Normals 3n, 3n+1, 3n+2 of particle p's stream in replica r, scaled by sig:
+// normal j is component j % 4 of Box-Muller over Philox call j / 4 (the
+// stream of philox.h).  Only the Box-Muller pairs that hold the three are
+// formed: the two of call 3n/4 when they start at component 0 or 1, else the
+// second pair of that call and the first of the next.
+__device__ inline void replica_normals(const ReplicaArgs &a, int r, int p, int64_t n, float sig,
+                                       float &sx, float &sy, float &sz)
+{
+    PhiloxKeys keys;
+    keys.init(a.key0[r], a.key1[r]);
+    const uint64_t j0 = 3 * (uint64_t)n, c0 = j0 >> 2;
+    const int o = (int)(j0 & 3);
+    const u32x4 w = philox4x32_10(u32x4{(uint32_t)c0, (uint32_t)(c0 >> 32), (uint32_t)p, a.tag}, keys);
+    float z0, z1, z2;
+    if (o < 2) {
+        float b0, b1, b2, b3;
+        Num<float>::box_muller(w.x, w.y, b0, b1);
+        Num<float>::box_muller(w.z, w.w, b2, b3);
+        z0 = o == 0 ? b0 : b1;
+        z1 = o == 0 ? b1 : b2;
+        z2 = o == 0 ? b2 : b3;
+    } else {
+        const uint64_t c1 = c0 + 1;
+        const u32x4 w1 =
+            philox4x32_10(u32x4{(uint32_t)c1, (uint32_t)(c1 >> 32), (uint32_t)p, a.tag}, keys);
+        float b2, b3, b4, b5;
+        Num<float>::box_muller(w.z, w.w, b2, b3);
+        Num<float>::box_muller(w1.x, w1.y, b4, b5);
+        z0 = o == 2 ? b2 : b3;
+        z1 = o == 2 ? b3 : b4;
+        z2 = o == 2 ? b4 : b5;
+    }
+    sx = sig * z0;
+    sy = sig * z1;
+    sz = sig * z2;
+}
+
+// One element (replica r, particle p at column ox of its x component) of
+// the Heun stages; the field (hx, hy, hz) is the product's output there.
+// Stage 1: m~ = m + L(m, h dt + s), stored to mt (the next product's operand).
+__device__ inline void replica_stage1_elem(const ReplicaArgs &a, int r, int p, size_t ox,
+                                           int64_t n_step, float h1, float hx, float hy, float hz)
+{
+    const size_t oy = ox + 32, oz = ox + 64;
+    const float mx = a.m[ox], my = a.m[oy], mz = a.m[oz];
+    float sx, sy, sz;
+    replica_normals(a, r, p, n_step, a.sig[p], sx, sy, sz);
+    const float dt = a.dt;
+    const float d = mx * a.kex[p] + my * a.key[p] + mz * a.kez[p];
+    const float vx = d * a.edx[p] + (hx * dt + sx), vy = d * a.edy[p] + (hy * dt + sy),
+                vz = d * a.edz[p] + ((hz + h1) * dt + sz);
+    float ax, ay, az;
+    llg_map(mx, my, mz, vx, vy, vz, a.alpha, ax, ay, az);
+    a.mt[ox] = mx + ax;
+    a.mt[oy] = my + ay;
+    a.mt[oz] = mz + az;
+}
+
+// Stage 2: m' = m + (L(m, v1) + L(m~, v2)) / 2 with L(m, v1) = m~ - m; stores
+// m' (live elements only) and returns it; flags a non-finite state.
+__device__ inline void replica_stage2_elem(const ReplicaArgs &a, int r, int p, size_t ox,
+                                           int64_t n_step, float h2, float hx, float hy, float hz,
+                                           bool live, float &mx, float &my, float &mz)
+{
+    const size_t oy = ox + 32, oz = ox + 64;
+    const float tx = a.mt[ox], ty = a.mt[oy], tz = a.mt[oz];
+    float sx, sy, sz;
+    replica_normals(a, r, p, n_step, a.sig[p], sx, sy, sz);
+    const float dt = a.dt;
+    const float d = tx * a.kex[p] + ty * a.key[p] + tz * a.kez[p];
+    const float vx = d * a.edx[p] + (hx * dt + sx), vy = d * a.edy[p] + (hy * dt + sy),
+                vz = d * a.edz[p] + ((hz + h2) * dt + sz);
+    float bx, by, bz;
+    llg_map(tx, ty, tz, vx, vy, vz, a.alpha, bx, by, bz);
+    const float m0x = a.m[ox], m0y = a.m[oy], m0z = a.m[oz];
+    mx = m0x + 0.5f * ((tx - m0x) + bx);
+    my = m0y + 0.5f * ((ty - m0y) + by);
+    mz = m0z + 0.5f * ((tz - m0z) + bz);
+    if (a.renorm) {
+        const float nrm = sqrtf(mx * mx + my * my + mz * mz);
+        mx = mx / nrm;
+        my = my / nrm;
+        mz = mz / nrm;
+    }
+    if (live) {
+        a.m[ox] = mx;
+        a.m[oy] = my;
+        a.m[oz] = mz;
+        if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[(size_t)r * a.n + p] |= 1u;
+    }
+}
+
 // Ht = Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T with LDS staging; Mt is
 // read in FP32 and split on its way to LDS, D comes pre-split.  Block
 // tile 128 rows x 192 columns; each tile of 64 x 96 (2 x 3 MFMA tiles) is
@@ -73,14 +191,21 @@ __device__ inline void store_tile32(float *C, int ldc, int r0, int c0, const f32
 // XCD's L2.  rows % 128 == 0, cols % 192 == 0, K % 192 == 0 (stages come in
 // sixes).  MFMAs are ordered so that six independent accumulators separate
 // dependent ones.
+//
+// kEpi selects the epilogue: 0 stores the product Ht to c; 1 and 2 run the
+// Heun stage 1 or 2 on the tile instead (ra, stage field hstage, step
+// n_step; stage 2 records the samples s_first .. s_first + n_rec - 1), so
+// the field never leaves the kernel.  A lane holds x, y and z of the same
+// (replica, particle) in its three column subtiles (u = 0, 1, 2), so the
+// stage is per lane; the two k halves each finish half of the registers
+// (kh = 0 the registers 0-7 of every subtile, kh = 1 the registers 8-15).
 constexpr int kBM = 128, kBN = 192, kBK = 32;
 
-static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(const float *__restrict__ a,
-                                                                 const __bf16 *__restrict__ b_hi,
-                                                                 const __bf16 *__restrict__ b_lo,
-                                                                 float *__restrict__ c, int K,
-                                                                 int ldc, int n_row_blocks,
-                                                                 int n_col_blocks)
+template <int kEpi>
+static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(
+    const float *__restrict__ a, const __bf16 *__restrict__ b_hi,
+    const __bf16 *__restrict__ b_lo, float *__restrict__ c, int K, int ldc, int n_row_blocks,
+    int n_col_blocks, ReplicaArgs ra, int64_t n_step, float hstage, int64_t s_first, int n_rec)
 {
     constexpr int kLdsRow = kBK + 8;
     constexpr int kPanelA = kBM * kLdsRow, kPanelB = kBN * kLdsRow;   // bf16 per panel
@@ -209,25 +334,102 @@ static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(const fl
     // of the two k halves (4 tiles x 6144 floats = 96 KB)
     float *red = reinterpret_cast<float *>(lds);
     static_assert(sizeof(lds) >= 4 * 6144 * sizeof(float), "reduction buffer");
-    float *mine = red + (size_t)w4 * 6144;
-    if (kh == 1) {
+    if constexpr (kEpi == 0) {
+        float *mine = red + (size_t)w4 * 6144;
+        if (kh == 1) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+            for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int u = 0; u < 3; ++u)
+                for (int u = 0; u < 3; ++u)
 #pragma unroll
-                for (int q = 0; q < 16; ++q) mine[((t * 3 + u) * 16 + q) * 64 + l] = acc[t][u][q];
-    }
-    __syncthreads();
-    if (kh == 0) {
+                    for (int q = 0; q < 16; ++q) mine[((t * 3 + u) * 16 + q) * 64 + l] = acc[t][u][q];
+        }
+        __syncthreads();
+        if (kh == 0) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+            for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int u = 0; u < 3; ++u) {
+                for (int u = 0; u < 3; ++u) {
 #pragma unroll
-                for (int q = 0; q < 16; ++q) acc[t][u][q] += mine[((t * 3 + u) * 16 + q) * 64 + l];
-                store_tile32(c, ldc, row0 + 64 * wr + 32 * t, col0 + 96 * wc + 32 * u, acc[t][u]);
+                    for (int q = 0; q < 16; ++q) acc[t][u][q] += mine[((t * 3 + u) * 16 + q) * 64 + l];
+                    store_tile32(c, ldc, row0 + 64 * wr + 32 * t, col0 + 96 * wc + 32 * u, acc[t][u]);
+                }
+        }
+    } else {
+        // each half sends the registers the other finishes
+        float *send = red + (size_t)(2 * w4 + kh) * 3072;
+        const float *recv = red + (size_t)(2 * w4 + 1 - kh) * 3072;
+        auto send_half = [&](auto qs_c) {
+            constexpr int QS = decltype(qs_c)::value;
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 3; ++u)
+#pragma unroll
+                    for (int qq = 0; qq < 8; ++qq)
+                        send[((t * 3 + u) * 8 + qq) * 64 + l] = acc[t][u][QS + qq];
+        };
+        if (kh == 0) send_half(std::integral_constant<int, 8>{});
+        else send_half(std::integral_constant<int, 0>{});
+        __syncthreads();
+        const int gcol = col0 + 96 * wc;             // x column of this wave's 32 particles
+        const int p = 32 * (gcol / 96) + (l & 31);   // particle of this lane's columns
+        double ex = 0.0, ey = 0.0, ez = 0.0;
+        auto finish_half = [&](auto qm_c) {
+            constexpr int QM = decltype(qm_c)::value;
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int qq = 0; qq < 8; ++qq) {
+                    const int q = QM + qq;
+                    float h[3];
+#pragma unroll
+                    for (int u = 0; u < 3; ++u)
+                        h[u] = acc[t][u][q] + recv[((t * 3 + u) * 8 + qq) * 64 + l];
+                    const int r = row0 + 64 * wr + 32 * t + 8 * (q >> 2) + 4 * (l >> 5) + (q & 3);
+                    const bool live = r < ra.reps && p < ra.n;
+                    if (!live) continue;
+                    const size_t ox = (size_t)r * ra.ld + gcol + (l & 31);
+                    if constexpr (kEpi == 1) {
+                        replica_stage1_elem(ra, r, p, ox, n_step, hstage, h[0], h[1], h[2]);
+                    } else {
+                        float mx, my, mz;
+                        replica_stage2_elem(ra, r, p, ox, n_step, hstage, h[0], h[1], h[2], true,
+                                            mx, my, mz);
+                        if (n_rec > 0) {
+                            const double w = ra.msw[r];
+                            ex += (double)mx * w;
+                            ey += (double)my * w;
+                            ez += (double)mz * w;
+                            if (ra.samples != nullptr) {
+                                const size_t P = (size_t)ra.n * ra.reps, i = (size_t)r * ra.n + p;
+                                for (int k = 0; k < n_rec; ++k) {
+                                    float *base = ra.samples + (size_t)(s_first + k) * 3 * P;
+                                    base[i] = mx;
+                                    base[P + i] = my;
+                                    base[2 * P + i] = mz;
+                                }
+                            }
+                        }
+                    }
+                }
+        };
+        if (kh == 0) finish_half(std::integral_constant<int, 0>{});
+        else finish_half(std::integral_constant<int, 8>{});
+        if constexpr (kEpi == 2) {
+            if (n_rec > 0 && ra.ens_partial != nullptr) {
+                wave_sum3<float>(ex, ey, ez);
+                if (l == 0) {
+                    const int wid = blockIdx.x * 8 + wave;
+                    for (int k = 0; k < n_rec; ++k) {
+                        double *dst = ra.ens_partial + (size_t)(s_first + k) * 3 * (size_t)ra.n_waves;
+                        dst[wid] = ex;
+                        dst[(size_t)ra.n_waves + wid] = ey;
+                        dst[2 * (size_t)ra.n_waves + wid] = ez;
+                    }
+                }
             }
+        }
     }
 }
 
@@ -289,80 +491,19 @@ static __global__ void gemm_f64_kernel(const double *__restrict__ a, const doubl
     c[(size_t)r * cols + col] = s;
 }
 
-// ------------------------------------------------------------------ Heun stages
-// Per-replica Heun stages around the product.  Thread = (replica r, particle
-// p), p fastest; state rows are replica-major with leading dimension ld
-// (>= 3N, padded).  The applied field of the two stage times is uniform and
-// passed in (h1, h2).
-struct ReplicaArgs {
-    int n;                  // particles per replica
-    int reps;               // replicas
-    int ld;                 // row length 3 Np (Np = n padded to a multiple of 128)
-    int n_waves;            // waves in the grid (ensemble partial stride)
-    const float *kex, *key, *kez, *edx, *edy, *edz;  // per particle: k e, e dt
-    const float *sig;       // per particle: sr sqrt(dt)
-    float dt, alpha;
-    const uint32_t *key0, *key1;   // per replica Philox key (member seed)
-    const double *msw;             // per replica Ms
-    uint32_t tag;
-    int renorm;
-    float *m, *mt, *h;             // [reps][ld]: state, predictor, field
-    uint32_t *status;              // [reps * n]
-    // sampling
-    const int64_t *sched;
-    int64_t S;
-    float *samples;                // [S][3][reps * n] or null
-    double *ens_partial;           // [S][3][n_waves] or null
-};
-
-__device__ inline void replica_normals(const ReplicaArgs &a, int r, int p, int64_t n, float sig,
-                                       float &sx, float &sy, float &sz)
-{
-    PhiloxKeys keys;
-    keys.init(a.key0[r], a.key1[r]);
-    float z[8];
-    const uint64_t c0 = (3 * (uint64_t)n) >> 2;
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const uint64_t c = c0 + i;
-        const u32x4 w = philox4x32_10(u32x4{(uint32_t)c, (uint32_t)(c >> 32), (uint32_t)p, a.tag}, keys);
-        Num<float>::box_muller(w.x, w.y, z[4 * i], z[4 * i + 1]);
-        Num<float>::box_muller(w.z, w.w, z[4 * i + 2], z[4 * i + 3]);
-    }
-    const int o = (int)((3 * (uint64_t)n) & 3);  // normals 3n..3n+2 start at z[o]
-    float w0 = z[0], w1 = z[1], w2 = z[2];
-    if (o == 1) { w0 = z[1]; w1 = z[2]; w2 = z[3]; }
-    else if (o == 2) { w0 = z[2]; w1 = z[3]; w2 = z[4]; }
-    else if (o == 3) { w0 = z[3]; w1 = z[4]; w2 = z[5]; }
-    sx = sig * w0;
-    sy = sig * w1;
-    sz = sig * w2;
-}
-
-// stage 1: m~ = m + L(m, h dt + s), stores m~ (the next product's operand)
+// The stages as elementwise kernels around an unfused product (FP32 MFMA
+// product): thread = (replica r, particle p), p fastest.
 static __global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs a, int64_t n_step, float h1)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n * a.reps) return;
     const int r = i / a.n, p = i % a.n;
-    const size_t ox = (size_t)r * a.ld + replica_col(p, 0), oy = ox + 32, oz = ox + 64;
-    const float mx = a.m[ox], my = a.m[oy], mz = a.m[oz];
-    const float hx = a.h[ox], hy = a.h[oy], hz = a.h[oz];
-    float sx, sy, sz;
-    replica_normals(a, r, p, n_step, a.sig[p], sx, sy, sz);
-    const float dt = a.dt;
-    const float d = mx * a.kex[p] + my * a.key[p] + mz * a.kez[p];
-    const float vx = d * a.edx[p] + (hx * dt + sx), vy = d * a.edy[p] + (hy * dt + sy),
-                vz = d * a.edz[p] + ((hz + h1) * dt + sz);
-    float ax, ay, az;
-    llg_map(mx, my, mz, vx, vy, vz, a.alpha, ax, ay, az);
-    a.mt[ox] = mx + ax;
-    a.mt[oy] = my + ay;
-    a.mt[oz] = mz + az;
+    const size_t ox = (size_t)r * a.ld + replica_col(p, 0);
+    replica_stage1_elem(a, r, p, ox, n_step, h1, a.h[ox], a.h[ox + 32], a.h[ox + 64]);
 }
 
-// stage 2: m' = m + (L(m, v1) + L(m~, v2)) / 2 with L(m, v1) = m~ - m;
-// stores m' and the samples
+// stage 2 and the samples s_first .. s_first + n_rec - 1 (all recorded after
+// this step)
 static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs a, int64_t n_step, float h2,
                                                              int64_t s_first, int n_rec)
 {
@@ -370,32 +511,10 @@ static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs 
     const bool live = i < a.n * a.reps;
     const int q = live ? i : a.n * a.reps - 1;
     const int r = q / a.n, p = q % a.n;
-    const size_t ox = (size_t)r * a.ld + replica_col(p, 0), oy = ox + 32, oz = ox + 64;
-    const float tx = a.mt[ox], ty = a.mt[oy], tz = a.mt[oz];
-    const float hx = a.h[ox], hy = a.h[oy], hz = a.h[oz];
-    float sx, sy, sz;
-    replica_normals(a, r, p, n_step, a.sig[p], sx, sy, sz);
-    const float dt = a.dt;
-    const float d = tx * a.kex[p] + ty * a.key[p] + tz * a.kez[p];
-    const float vx = d * a.edx[p] + (hx * dt + sx), vy = d * a.edy[p] + (hy * dt + sy),
-                vz = d * a.edz[p] + ((hz + h2) * dt + sz);
-    float bx, by, bz;
-    llg_map(tx, ty, tz, vx, vy, vz, a.alpha, bx, by, bz);
-    const float m0x = a.m[ox], m0y = a.m[oy], m0z = a.m[oz];
-    float mx = m0x + 0.5f * ((tx - m0x) + bx);
-    float my = m0y + 0.5f * ((ty - m0y) + by);
-    float mz = m0z + 0.5f * ((tz - m0z) + bz);
-    if (a.renorm) {
-        const float nrm = sqrtf(mx * mx + my * my + mz * mz);
-        mx = mx / nrm;
-        my = my / nrm;
-        mz = mz / nrm;
-    }
-    if (live) {
-        a.m[ox] = mx; a.m[oy] = my; a.m[oz] = mz;
-        if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[i] |= 1u;
-    }
-    // samples s_first .. s_first + n_rec - 1 are all recorded after this step
+    const size_t ox = (size_t)r * a.ld + replica_col(p, 0);
+    float mx, my, mz;
+    replica_stage2_elem(a, r, p, ox, n_step, h2, a.h[ox], a.h[ox + 32], a.h[ox + 64], live, mx,
+                        my, mz);
     for (int k = 0; k < n_rec; ++k) {
         const int64_t s = s_first + k;
         if (a.samples != nullptr && live) {
