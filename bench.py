@@ -709,7 +709,7 @@ def grad_check_scale(args):
     done, prev = 0, -1
     while done < n_steps and done != prev:
         prev = done
-        done = ses.advance(1 << 25)
+        done = ses.advance(1 << 24)
         print(json.dumps({"pass": "fd fp64", "progress_steps": int(done), "of": int(n_steps),
                           "elapsed_s": round(time.perf_counter() - t0, 1)}), flush=True)
     out = ses.finish()

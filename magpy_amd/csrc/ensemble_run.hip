@@ -1189,12 +1189,22 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
     h.ms.assign(sp.magnetisation, sp.magnetisation + sp.n_members);
     h.shape.assign(sp.field_shape, sp.field_shape + sp.n_members);
 
-    // reduce every member, schedule per member, group members by schedule
+    // reduce every member, schedule per member, group members by schedule.
+    // Every kernel reads its lane's reduced time step, so members whose
+    // integer schedules (sample steps and step count) are equal share one
+    // launch even when their reduced steps differ (other damping or
+    // anisotropy).  FP32 Heun keeps one group per reduced step: there a
+    // group whose lanes all see one applied field reads it from a shared
+    // table, whose bits differ from the per-lane evaluation, and a member's
+    // result must not depend on the rest of the ensemble.
+    const bool merge_by_schedule =
+        !(sp.precision == MP_PRECISION_FP32 && sp.scheme == MP_SCHEME_HEUN);
     plan.red.reserve((size_t)sp.n_members);
     plan.member_of.resize((size_t)P);
-    std::map<std::pair<double, double>, int> key_to_group;
-    std::vector<int> member_group((size_t)sp.n_members);
-    std::vector<std::pair<double, double>> group_keys;
+    std::map<std::pair<double, double>, int> step_key;        // (dt, end) -> schedule
+    std::vector<std::vector<int64_t>> scheds;                 // schedule + step count
+    std::map<std::vector<int64_t>, int> sched_to_group;
+    std::vector<int> member_group((size_t)sp.n_members), group_sched;
     for (int64_t m = 0; m < sp.n_members; ++m) {
         const int64_t b = sp.member_offsets[m], e = sp.member_offsets[m + 1];
         plan.red.push_back(simulation::reduce_member(
@@ -1203,14 +1213,29 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
             sp.field_frequency[m]));
         for (int64_t p = b; p < e; ++p) plan.member_of[(size_t)p] = m;
         const auto key = std::make_pair(plan.red.back().dt, plan.red.back().end_time);
-        auto it = key_to_group.find(key);
-        if (it == key_to_group.end()) {
-            it = key_to_group.emplace(key, (int)group_keys.size()).first;
-            group_keys.push_back(key);
+        auto it = step_key.find(key);
+        if (it == step_key.end()) {
+            std::vector<int64_t> k;
+            const int64_t steps =
+                simulation::sample_schedule(key.first, key.second, sp.max_samples, k);
+            k.push_back(steps);
+            it = step_key.emplace(key, (int)scheds.size()).first;
+            scheds.push_back(std::move(k));
         }
-        member_group[(size_t)m] = it->second;
+        const int si = it->second;
+        int g = -1;
+        if (merge_by_schedule) {
+            auto jt = sched_to_group.find(scheds[(size_t)si]);
+            if (jt == sched_to_group.end())
+                jt = sched_to_group.emplace(scheds[(size_t)si], (int)group_sched.size()).first;
+            g = jt->second;
+        } else {
+            g = si;   // groups are the distinct (dt, end) pairs in order
+        }
+        if (g == (int)group_sched.size()) group_sched.push_back(si);
+        member_group[(size_t)m] = g;
     }
-    const int G = (int)group_keys.size();
+    const int G = (int)group_sched.size();
     plan.group_begin.assign((size_t)G + 1, 0);
     for (int g = 0; g < G; ++g) {
         for (int64_t m = 0; m < sp.n_members; ++m)
@@ -1218,9 +1243,9 @@ mp_session *session_create(const simulation::EnsembleSpec &sp, bool traj, bool e
                 for (int64_t p = sp.member_offsets[m]; p < sp.member_offsets[m + 1]; ++p)
                     plan.perm.push_back(p);
         plan.group_begin[(size_t)g + 1] = (int64_t)plan.perm.size();
-        std::vector<int64_t> k;
-        const int64_t steps = simulation::sample_schedule(
-            group_keys[(size_t)g].first, group_keys[(size_t)g].second, sp.max_samples, k);
+        std::vector<int64_t> k = scheds[(size_t)group_sched[(size_t)g]];
+        const int64_t steps = k.back();
+        k.pop_back();
         plan.sched.push_back(std::move(k));
         plan.group_steps.push_back(steps);
     }

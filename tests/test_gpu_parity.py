@@ -174,6 +174,34 @@ def test_members_with_different_schedules():
             assert np.allclose(g[c][m], o[c][0], rtol=0, atol=4e5 * 1e-11)
 
 
+@pytest.mark.parametrize("implicit", [False, True])
+def test_members_with_different_steps_and_one_schedule(implicit):
+    # K and alpha moved by 1e-6 (a finite-difference ensemble): every member has
+    # its own reduced step but the same integer sample schedule, so the FP64
+    # run packs them into one launch; each equals its run alone, bit for bit
+    _print_case("heun_merged_schedules", implicit=implicit)
+    M, dt, S = 6, 1e-14, 17
+    end = 777.3 * dt      # no sample on an exact step: the schedules cannot split
+    h = np.array([0, 1, -1, 0, 0, 1]) * 1e-6
+    inp = ensemble_inputs(M, seed=8, K=23e3 * (1 + h))
+    inp["damping"] = 0.1 * (1 + np.array([0, 0, 0, 1, -1, -1]) * 1e-6)
+    kw = dict(precision="fp64", use_implicit=implicit, member_times=True)
+    g = run_gpu(inp, dt, end, S, **kw)
+    for m in range(M):
+        o = oracle_member(inp, m, dt, end, S, implicit=implicit)
+        assert np.array_equal(g["member_time"][m], o["time"])
+        for c in "xyz":
+            assert np.allclose(g[c][m], o[c][0], rtol=0, atol=4e5 * 1e-11), (m, c)
+        one = {k: (v[m:m + 1] if k in ("radius", "anisotropy", "anisotropy_axis",
+                                       "magnetisation_direction", "location", "seeds")
+                   else v) for k, v in inp.items()}
+        one["member_offsets"] = np.array([0, 1], dtype=np.int64)
+        one["damping"] = inp["damping"][m]
+        g1 = run_gpu(one, dt, end, S, **kw)
+        for c in "xyz":
+            np.testing.assert_array_equal(g1[c][0], g[c][m])
+
+
 def test_ensemble_sum_matches_trajectories():
     _print_case("ensemble_sum")
     M, dt, S = 300, 1e-14, 9
