@@ -756,6 +756,9 @@ def grad_check_scale(args):
             "checked_sum_rel_err_fp64": (np.abs(adj.sum(0) - fdc.sum(0)) /
                                          np.abs(fdc.sum(0))).tolist(),
             "status": int(np.max(out["status"]))}
+    # the accuracy criterion: the checked particles' summed gradient within
+    # 1e-3 of the summed central differences, every component
+    line["sum_within_1e-3_fp64"] = bool(np.all(np.array(line["checked_sum_rel_err_fp64"]) < 1e-3))
     if a32 is not None:
         adj32, err32 = rel(a32["particle_grad_sar"][sel])
         g32 = a32["grad_sar"]
@@ -764,6 +767,9 @@ def grad_check_scale(args):
         line["max_rel_err_fp32_top"] = err32[:nt].max(axis=0).tolist()
         line["max_rel_err_fp32_random"] = err32[nt:].max(axis=0).tolist()
         line["median_rel_err_fp32_random"] = np.median(err32[nt:], axis=0).tolist()
+        sum32 = np.abs(adj32.sum(0) - fdc.sum(0)) / np.abs(fdc.sum(0))
+        line["checked_sum_rel_err_fp32"] = sum32.tolist()
+        line["sum_within_1e-3_fp32"] = bool(np.all(sum32 < 1e-3))
     np.savez(args.grad_check_out or os.path.join(ROOT, "gpurun_out", "grad_scale_fd.npz"),
              sel=sel, fd=fd, adj=adj, sar_i=sar_i)
     print(json.dumps(line), flush=True)
