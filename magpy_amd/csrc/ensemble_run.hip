@@ -665,6 +665,29 @@ void split_host(const std::vector<double> &x, std::vector<__bf16> &hi, std::vect
 int replica_ld(int64_t n) { return 3 * pad_to(n, mp::kParticlePad); }
 int replica_rows(int64_t reps) { return pad_to(reps, mp::kBM); }
 
+// Which panels the blocks that one XCD runs share: 1 (default) the row
+// panels of the state, so each XCD reads an eighth of it and all of the
+// tensor (100 MB of L2 fills per 4096 x 1536 product instead of 209 MB);
+// 0 (MAGPY_AMD_REPLICA_PANELS=cols) the column panels of the tensor.
+int replica_row_panels()
+{
+    static const int v = [] {
+        const char *e = std::getenv("MAGPY_AMD_REPLICA_PANELS");
+        return e != nullptr && std::string(e) == "cols" ? 0 : 1;
+    }();
+    return v;
+}
+
+// The BF16x3 product runs the Heun stages in its epilogue unless
+// MAGPY_AMD_REPLICA_FUSED=0 when the run is created (then it writes the
+// field and separate stage kernels follow, as for the FP32 product; the two
+// give the same bits).
+bool replica_fused()
+{
+    const char *e = std::getenv("MAGPY_AMD_REPLICA_FUSED");
+    return !(e != nullptr && std::string(e) == "0");
+}
+
 void launch_product(int mode, const float *a32, const float *b32, const __bf16 *b_hi,
                     const __bf16 *b_lo, float *c, int rows, int ld, hipStream_t s)
 {
@@ -678,7 +701,7 @@ void launch_product(int mode, const float *a32, const float *b32, const __bf16 *
         const int nrb = rows / mp::kBM, ncb = ld / mp::kBN;
         hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel<0>, dim3((unsigned)(nrb * ncb)), dim3(512),
                            0, s, a32, b_hi, b_lo, c, ld, ld, nrb, ncb, mp::ReplicaArgs{},
-                           (int64_t)0, 0.f, (int64_t)0, 0);
+                           (int64_t)0, 0.f, (int64_t)0, 0, replica_row_panels());
     }
 }
 
@@ -687,12 +710,13 @@ void launch_product(int mode, const float *a32, const float *b32, const __bf16 *
 // products read the FP32 state and split it to BF16 themselves.
 struct ReplicaRun : GroupBase {
     int n = 0, reps = 0, ld = 0, rows = 0, mode = MP_DIPOLAR_GEMM_BF16X3, n_waves = 0;
+    bool fused = false;
     int shape = MP_FIELD_CONSTANT;
     double happ = 0.0, fapp = 0.0, dt = 0.0;
     bool want_traj = false, want_ens = false;
     int64_t S = 0, s_next = 0;
     std::vector<int64_t> sched, slot_particle;
-    DevBuf<float> d_m, d_mt, d_h, d_kex, d_key, d_kez, d_edx, d_edy, d_edz, d_sig, d_b32,
+    DevBuf<float> d_m, d_mt, d_h, d_nz, d_kex, d_key, d_kez, d_edx, d_edy, d_edz, d_sig, d_b32,
         d_samples;
     DevBuf<__bf16> d_bhi, d_blo;
     DevBuf<uint32_t> d_key0, d_key1, d_status;
@@ -703,6 +727,7 @@ struct ReplicaRun : GroupBase {
     ReplicaRun(const simulation::EnsembleSpec &sp, const Plan &plan, bool traj, bool ens, int m)
     {
         mode = m;
+        fused = mode == MP_DIPOLAR_GEMM_BF16X3 && replica_fused();
         reps = (int)sp.n_members;
         n = (int)(sp.member_offsets[1] - sp.member_offsets[0]);
         ld = replica_ld(n);
@@ -772,13 +797,16 @@ struct ReplicaRun : GroupBase {
         n_done = std::min(sp.start_step, n_total);
         d_m.upload(cast_vec<float>(m0));
         d_mt.upload(std::vector<float>(m0.size(), 0.f));
-        d_h.upload(std::vector<float>(m0.size(), 0.f));
+        // the fused product keeps the field on chip and passes the step's
+        // increments from stage 1 to stage 2; the separate stages pass the field
+        if (fused) d_nz.upload(std::vector<float>(m0.size(), 0.f));
+        else d_h.upload(std::vector<float>(m0.size(), 0.f));
         d_status.upload(std::vector<uint32_t>((size_t)reps * n, 0));
         d_sched.upload(sched);
         const int threads = reps * n;
         const int grid = (threads + 255) / 256;
         // the fused BF16x3 stages write one ensemble partial per product wave
-        n_waves = mode == MP_DIPOLAR_GEMM_BF16X3 ? (rows / mp::kBM) * (ld / mp::kBN) * 8 : grid * 4;
+        n_waves = fused ? (rows / mp::kBM) * (ld / mp::kBN) * 8 : grid * 4;
         if (want_traj) d_samples.alloc((size_t)S * 3 * (size_t)threads);
         if (want_ens) {
             d_partial.alloc((size_t)S * 3 * (size_t)n_waves);
@@ -797,7 +825,7 @@ struct ReplicaRun : GroupBase {
         a.msw = d_msw.p;
         a.tag = sp.noise_tag;
         a.renorm = sp.renorm ? 1 : 0;
-        a.m = d_m.p; a.mt = d_mt.p; a.h = d_h.p;
+        a.m = d_m.p; a.mt = d_mt.p; a.h = d_h.p; a.nz = d_nz.p;
         a.status = d_status.p;
         a.sched = d_sched.p;
         a.S = S;
@@ -820,7 +848,7 @@ struct ReplicaRun : GroupBase {
             while (s_next < S && sched[(size_t)s_next] < st + 1) ++s_next;
             int n_rec = 0;
             while (s_next + n_rec < S && sched[(size_t)(s_next + n_rec)] == st + 1) ++n_rec;
-            if (mode == MP_DIPOLAR_GEMM_BF16X3) {
+            if (fused) {
                 // the stages run in the products' epilogues
                 require(rows % mp::kBM == 0 && ld % mp::kBN == 0 && ld % (6 * mp::kBK) == 0,
                         "replica product: rows or ld do not fit the tiles");
@@ -828,10 +856,10 @@ struct ReplicaRun : GroupBase {
                 const dim3 pg((unsigned)(nrb * ncb)), pb(512);
                 hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel<1>, pg, pb, 0, s, d_m.p, d_bhi.p,
                                    d_blo.p, (float *)nullptr, ld, ld, nrb, ncb, a, st, h1,
-                                   (int64_t)0, 0);
+                                   (int64_t)0, 0, replica_row_panels());
                 hipLaunchKernelGGL(mp::gemm_bf16x3_ksplit_kernel<2>, pg, pb, 0, s, d_mt.p, d_bhi.p,
                                    d_blo.p, (float *)nullptr, ld, ld, nrb, ncb, a, st, h2, s_next,
-                                   n_rec);
+                                   n_rec, replica_row_panels());
             } else {
                 launch_product(mode, d_m.p, d_b32.p, d_bhi.p, d_blo.p, d_h.p, rows, ld, s);
                 hipLaunchKernelGGL(mp::replica_stage1_kernel, grid, dim3(256), 0, s, a, st, h1);

@@ -79,6 +79,8 @@ struct ReplicaArgs {
     uint32_t tag;
     int renorm;
     float *m, *mt, *h;             // [reps][ld]: state, predictor, field
+    float *nz;                     // [reps][ld]: the step's scaled Wiener increments (fused
+                                   // product: stage 1 writes them, stage 2 reads them)
     uint32_t *status;              // [reps * n]
     // sampling
     const int64_t *sched;
@@ -87,19 +89,20 @@ struct ReplicaArgs {
     double *ens_partial;           // [S][3][n_waves] or null
 };
 
-// Normals 3n, 3n+1, 3n+2 of particle p's stream in replica r, scaled by sig:
-// normal j is component j % 4 of Box-Muller over Philox call j / 4 (the
-// stream of philox.h).  Only the Box-Muller pairs that hold the three are
-// formed: the two of call 3n/4 when they start at component 0 or 1, else the
-// second pair of that call and the first of the next.
-__device__ inline void replica_normals(const ReplicaArgs &a, int r, int p, int64_t n, float sig,
-                                       float &sx, float &sy, float &sz)
+// Normals 3n, 3n+1, 3n+2 of particle p's stream under the Philox key
+// (k0, k1) of its replica, scaled by sig: normal j is component j % 4 of
+// Box-Muller over Philox call j / 4 (the stream of philox.h).  Only the
+// Box-Muller pairs that hold the three are formed: the two of call 3n/4 when
+// they start at component 0 or 1, else the second pair of that call and the
+// first of the next.
+__device__ inline void replica_normals(uint32_t k0, uint32_t k1, uint32_t tag, int p, int64_t n,
+                                       float sig, float &sx, float &sy, float &sz)
 {
     PhiloxKeys keys;
-    keys.init(a.key0[r], a.key1[r]);
+    keys.init(k0, k1);
     const uint64_t j0 = 3 * (uint64_t)n, c0 = j0 >> 2;
     const int o = (int)(j0 & 3);
-    const u32x4 w = philox4x32_10(u32x4{(uint32_t)c0, (uint32_t)(c0 >> 32), (uint32_t)p, a.tag}, keys);
+    const u32x4 w = philox4x32_10(u32x4{(uint32_t)c0, (uint32_t)(c0 >> 32), (uint32_t)p, tag}, keys);
     float z0, z1, z2;
     if (o < 2) {
         float b0, b1, b2, b3;
@@ -111,7 +114,7 @@ __device__ inline void replica_normals(const ReplicaArgs &a, int r, int p, int64
     } else {
         const uint64_t c1 = c0 + 1;
         const u32x4 w1 =
-            philox4x32_10(u32x4{(uint32_t)c1, (uint32_t)(c1 >> 32), (uint32_t)p, a.tag}, keys);
+            philox4x32_10(u32x4{(uint32_t)c1, (uint32_t)(c1 >> 32), (uint32_t)p, tag}, keys);
         float b2, b3, b4, b5;
         Num<float>::box_muller(w.z, w.w, b2, b3);
         Num<float>::box_muller(w1.x, w1.y, b4, b5);
@@ -124,44 +127,52 @@ __device__ inline void replica_normals(const ReplicaArgs &a, int r, int p, int64
     sz = sig * z2;
 }
 
-// One element (replica r, particle p at column ox of its x component) of
-// the Heun stages; the field (hx, hy, hz) is the product's output there.
-// Stage 1: m~ = m + L(m, h dt + s), stored to mt (the next product's operand).
-__device__ inline void replica_stage1_elem(const ReplicaArgs &a, int r, int p, size_t ox,
-                                           int64_t n_step, float h1, float hx, float hy, float hz)
+// Per-particle constants of the stages: k e, e dt and sr sqrt(dt).
+struct ReplicaParticle {
+    float kex, key, kez, edx, edy, edz, sig;
+};
+
+__device__ inline ReplicaParticle replica_particle(const ReplicaArgs &a, int p)
 {
-    const size_t oy = ox + 32, oz = ox + 64;
-    const float mx = a.m[ox], my = a.m[oy], mz = a.m[oz];
-    float sx, sy, sz;
-    replica_normals(a, r, p, n_step, a.sig[p], sx, sy, sz);
-    const float dt = a.dt;
-    const float d = mx * a.kex[p] + my * a.key[p] + mz * a.kez[p];
-    const float vx = d * a.edx[p] + (hx * dt + sx), vy = d * a.edy[p] + (hy * dt + sy),
-                vz = d * a.edz[p] + ((hz + h1) * dt + sz);
-    float ax, ay, az;
-    llg_map(mx, my, mz, vx, vy, vz, a.alpha, ax, ay, az);
-    a.mt[ox] = mx + ax;
-    a.mt[oy] = my + ay;
-    a.mt[oz] = mz + az;
+    return ReplicaParticle{a.kex[p], a.key[p], a.kez[p], a.edx[p], a.edy[p], a.edz[p], a.sig[p]};
 }
 
-// Stage 2: m' = m + (L(m, v1) + L(m~, v2)) / 2 with L(m, v1) = m~ - m; stores
-// m' (live elements only) and returns it; flags a non-finite state.
-__device__ inline void replica_stage2_elem(const ReplicaArgs &a, int r, int p, size_t ox,
-                                           int64_t n_step, float h2, float hx, float hy, float hz,
-                                           bool live, float &mx, float &my, float &mz)
+// The stage arithmetic of one (replica, particle) element from values
+// already in registers: the field (hx, hy, hz) is the product's output,
+// (sx, sy, sz) the step's scaled Wiener increments (replica_normals).  Both
+// the fused epilogue and the elementwise kernels evaluate these same
+// expressions.
+// Stage 1: m~ = m + L(m, h dt + s).
+__device__ inline void replica_stage1_math(const ReplicaArgs &a, const ReplicaParticle &c,
+                                           float sx, float sy, float sz, float h1, float hx,
+                                           float hy, float hz, float mx, float my, float mz,
+                                           float &tx, float &ty, float &tz)
 {
-    const size_t oy = ox + 32, oz = ox + 64;
-    const float tx = a.mt[ox], ty = a.mt[oy], tz = a.mt[oz];
-    float sx, sy, sz;
-    replica_normals(a, r, p, n_step, a.sig[p], sx, sy, sz);
     const float dt = a.dt;
-    const float d = tx * a.kex[p] + ty * a.key[p] + tz * a.kez[p];
-    const float vx = d * a.edx[p] + (hx * dt + sx), vy = d * a.edy[p] + (hy * dt + sy),
-                vz = d * a.edz[p] + ((hz + h2) * dt + sz);
+    const float d = mx * c.kex + my * c.key + mz * c.kez;
+    const float vx = d * c.edx + (hx * dt + sx), vy = d * c.edy + (hy * dt + sy),
+                vz = d * c.edz + ((hz + h1) * dt + sz);
+    float ax, ay, az;
+    llg_map(mx, my, mz, vx, vy, vz, a.alpha, ax, ay, az);
+    tx = mx + ax;
+    ty = my + ay;
+    tz = mz + az;
+}
+
+// Stage 2: m' = m + (L(m, v1) + L(m~, v2)) / 2 with L(m, v1) = m~ - m,
+// renormalised when asked.
+__device__ inline void replica_stage2_math(const ReplicaArgs &a, const ReplicaParticle &c,
+                                           float sx, float sy, float sz, float h2, float hx,
+                                           float hy, float hz, float tx, float ty, float tz,
+                                           float m0x, float m0y, float m0z, float &mx, float &my,
+                                           float &mz)
+{
+    const float dt = a.dt;
+    const float d = tx * c.kex + ty * c.key + tz * c.kez;
+    const float vx = d * c.edx + (hx * dt + sx), vy = d * c.edy + (hy * dt + sy),
+                vz = d * c.edz + ((hz + h2) * dt + sz);
     float bx, by, bz;
     llg_map(tx, ty, tz, vx, vy, vz, a.alpha, bx, by, bz);
-    const float m0x = a.m[ox], m0y = a.m[oy], m0z = a.m[oz];
     mx = m0x + 0.5f * ((tx - m0x) + bx);
     my = m0y + 0.5f * ((ty - m0y) + by);
     mz = m0z + 0.5f * ((tz - m0z) + bz);
@@ -171,12 +182,16 @@ __device__ inline void replica_stage2_elem(const ReplicaArgs &a, int r, int p, s
         my = my / nrm;
         mz = mz / nrm;
     }
-    if (live) {
-        a.m[ox] = mx;
-        a.m[oy] = my;
-        a.m[oz] = mz;
-        if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[(size_t)r * a.n + p] |= 1u;
-    }
+}
+
+// Store m' of a live element and flag a non-finite state.
+__device__ inline void replica_store_state(const ReplicaArgs &a, int r, int p, size_t ox, float mx,
+                                           float my, float mz)
+{
+    a.m[ox] = mx;
+    a.m[ox + 32] = my;
+    a.m[ox + 64] = mz;
+    if (!(isfinite(mx) && isfinite(my) && isfinite(mz))) a.status[(size_t)r * a.n + p] |= 1u;
 }
 
 // Ht = Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T with LDS staging; Mt is
@@ -205,7 +220,8 @@ template <int kEpi>
 static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(
     const float *__restrict__ a, const __bf16 *__restrict__ b_hi,
     const __bf16 *__restrict__ b_lo, float *__restrict__ c, int K, int ldc, int n_row_blocks,
-    int n_col_blocks, ReplicaArgs ra, int64_t n_step, float hstage, int64_t s_first, int n_rec)
+    int n_col_blocks, ReplicaArgs ra, int64_t n_step, float hstage, int64_t s_first, int n_rec,
+    int row_panels)
 {
     constexpr int kLdsRow = kBK + 8;
     constexpr int kPanelA = kBM * kLdsRow, kPanelB = kBN * kLdsRow;   // bf16 per panel
@@ -219,7 +235,8 @@ static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(
     const int total = n_row_blocks * n_col_blocks;
     int logical = blockIdx.x;
     if (total % 8 == 0) logical = (blockIdx.x % 8) * (total / 8) + blockIdx.x / 8;
-    const int cb = logical / n_row_blocks, rb = logical % n_row_blocks;
+    const int cb = row_panels ? logical % n_col_blocks : logical / n_row_blocks;
+    const int rb = row_panels ? logical / n_col_blocks : logical % n_row_blocks;
     const int row0 = rb * kBM, col0 = cb * kBN;
     f32x16 acc[2][3];
 #pragma unroll
@@ -356,7 +373,11 @@ static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(
                 }
         }
     } else {
-        // each half sends the registers the other finishes
+        // each half sends the registers the other finishes.  The stage's
+        // loads of a lane's 8 elements per row tile (stage 1: state and
+        // Philox keys; stage 2: state, predictor and the increments stage 1
+        // drew) are issued together, the first tile's before the exchange
+        // barrier.
         float *send = red + (size_t)(2 * w4 + kh) * 3072;
         const float *recv = red + (size_t)(2 * w4 + 1 - kh) * 3072;
         auto send_half = [&](auto qs_c) {
@@ -369,53 +390,102 @@ static __global__ __launch_bounds__(512) void gemm_bf16x3_ksplit_kernel(
                     for (int qq = 0; qq < 8; ++qq)
                         send[((t * 3 + u) * 8 + qq) * 64 + l] = acc[t][u][QS + qq];
         };
-        if (kh == 0) send_half(std::integral_constant<int, 8>{});
-        else send_half(std::integral_constant<int, 0>{});
-        __syncthreads();
         const int gcol = col0 + 96 * wc;             // x column of this wave's 32 particles
         const int p = 32 * (gcol / 96) + (l & 31);   // particle of this lane's columns
+        const ReplicaParticle cp = replica_particle(ra, p < ra.n ? p : ra.n - 1);
+        const int qm = 8 * kh;                       // registers this half finishes
+        struct Batch {
+            float m[8][3], t[8][3], z[8][3];
+            uint32_t k0[8], k1[8];
+        };
+        auto row_of = [&](int t, int qq) {
+            const int q = qm + qq;
+            return row0 + 64 * wr + 32 * t + 8 * (q >> 2) + 4 * (l >> 5) + (q & 3);
+        };
+        auto load_batch = [&](int t, Batch &bt) {
+#pragma unroll
+            for (int qq = 0; qq < 8; ++qq) {
+                const int r = row_of(t, qq);
+                const int rr = r < ra.reps ? r : 0;      // rows past the last replica: row 0
+                const size_t ox = (size_t)rr * ra.ld + gcol + (l & 31);
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    bt.m[qq][u] = ra.m[ox + 32 * u];
+                    if constexpr (kEpi == 2) {
+                        bt.t[qq][u] = ra.mt[ox + 32 * u];
+                        bt.z[qq][u] = ra.nz[ox + 32 * u];
+                    }
+                }
+                if constexpr (kEpi == 1) {
+                    bt.k0[qq] = ra.key0[rr];
+                    bt.k1[qq] = ra.key1[rr];
+                }
+            }
+        };
         double ex = 0.0, ey = 0.0, ez = 0.0;
-        auto finish_half = [&](auto qm_c) {
-            constexpr int QM = decltype(qm_c)::value;
+        auto finish_batch = [&](auto qm_c, auto t_c, const Batch &bt) {
+            constexpr int QM = decltype(qm_c)::value, t = decltype(t_c)::value;
 #pragma unroll
-            for (int t = 0; t < 2; ++t)
+            for (int qq = 0; qq < 8; ++qq) {
+                float h[3];
 #pragma unroll
-                for (int qq = 0; qq < 8; ++qq) {
-                    const int q = QM + qq;
-                    float h[3];
-#pragma unroll
-                    for (int u = 0; u < 3; ++u)
-                        h[u] = acc[t][u][q] + recv[((t * 3 + u) * 8 + qq) * 64 + l];
-                    const int r = row0 + 64 * wr + 32 * t + 8 * (q >> 2) + 4 * (l >> 5) + (q & 3);
-                    const bool live = r < ra.reps && p < ra.n;
-                    if (!live) continue;
-                    const size_t ox = (size_t)r * ra.ld + gcol + (l & 31);
-                    if constexpr (kEpi == 1) {
-                        replica_stage1_elem(ra, r, p, ox, n_step, hstage, h[0], h[1], h[2]);
-                    } else {
-                        float mx, my, mz;
-                        replica_stage2_elem(ra, r, p, ox, n_step, hstage, h[0], h[1], h[2], true,
-                                            mx, my, mz);
-                        if (n_rec > 0) {
-                            const double w = ra.msw[r];
-                            ex += (double)mx * w;
-                            ey += (double)my * w;
-                            ez += (double)mz * w;
-                            if (ra.samples != nullptr) {
-                                const size_t P = (size_t)ra.n * ra.reps, i = (size_t)r * ra.n + p;
-                                for (int k = 0; k < n_rec; ++k) {
-                                    float *base = ra.samples + (size_t)(s_first + k) * 3 * P;
-                                    base[i] = mx;
-                                    base[P + i] = my;
-                                    base[2 * P + i] = mz;
-                                }
+                for (int u = 0; u < 3; ++u)
+                    h[u] = acc[t][u][QM + qq] + recv[((t * 3 + u) * 8 + qq) * 64 + l];
+                const int r = row_of(t, qq);
+                const bool live = r < ra.reps && p < ra.n;
+                if (!live) continue;
+                const size_t ox = (size_t)r * ra.ld + gcol + (l & 31);
+                if constexpr (kEpi == 1) {
+                    float sx, sy, sz, tx, ty, tz;
+                    replica_normals(bt.k0[qq], bt.k1[qq], ra.tag, p, n_step, cp.sig, sx, sy, sz);
+                    replica_stage1_math(ra, cp, sx, sy, sz, hstage, h[0], h[1], h[2], bt.m[qq][0],
+                                        bt.m[qq][1], bt.m[qq][2], tx, ty, tz);
+                    ra.mt[ox] = tx;
+                    ra.mt[ox + 32] = ty;
+                    ra.mt[ox + 64] = tz;
+                    ra.nz[ox] = sx;
+                    ra.nz[ox + 32] = sy;
+                    ra.nz[ox + 64] = sz;
+                } else {
+                    float mx, my, mz;
+                    replica_stage2_math(ra, cp, bt.z[qq][0], bt.z[qq][1], bt.z[qq][2], hstage, h[0],
+                                        h[1], h[2], bt.t[qq][0], bt.t[qq][1], bt.t[qq][2],
+                                        bt.m[qq][0], bt.m[qq][1], bt.m[qq][2], mx, my, mz);
+                    replica_store_state(ra, r, p, ox, mx, my, mz);
+                    if (n_rec > 0) {
+                        const double w = ra.msw[r];
+                        ex += (double)mx * w;
+                        ey += (double)my * w;
+                        ez += (double)mz * w;
+                        if (ra.samples != nullptr) {
+                            const size_t P = (size_t)ra.n * ra.reps, i = (size_t)r * ra.n + p;
+                            for (int k = 0; k < n_rec; ++k) {
+                                float *base = ra.samples + (size_t)(s_first + k) * 3 * P;
+                                base[i] = mx;
+                                base[P + i] = my;
+                                base[2 * P + i] = mz;
                             }
                         }
                     }
                 }
+            }
         };
-        if (kh == 0) finish_half(std::integral_constant<int, 0>{});
-        else finish_half(std::integral_constant<int, 8>{});
+        Batch b0, b1;
+        load_batch(0, b0);
+        if (kh == 0) send_half(std::integral_constant<int, 8>{});
+        else send_half(std::integral_constant<int, 0>{});
+        __syncthreads();
+        // stage 1 holds both tiles' loads in flight; stage 2 loads twice as
+        // much per tile and issues the second tile's after the first
+        if constexpr (kEpi == 1) load_batch(1, b1);
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I8 = std::integral_constant<int, 8>;
+        if (kh == 0) finish_batch(I0{}, I0{}, b0);
+        else finish_batch(I8{}, I0{}, b0);
+        if constexpr (kEpi == 2) load_batch(1, b1);
+        if (kh == 0) finish_batch(I0{}, I1{}, b1);
+        else finish_batch(I8{}, I1{}, b1);
         if constexpr (kEpi == 2) {
             if (n_rec > 0 && ra.ens_partial != nullptr) {
                 wave_sum3<float>(ex, ey, ez);
@@ -499,7 +569,14 @@ static __global__ __launch_bounds__(256) void replica_stage1_kernel(ReplicaArgs 
     if (i >= a.n * a.reps) return;
     const int r = i / a.n, p = i % a.n;
     const size_t ox = (size_t)r * a.ld + replica_col(p, 0);
-    replica_stage1_elem(a, r, p, ox, n_step, h1, a.h[ox], a.h[ox + 32], a.h[ox + 64]);
+    const ReplicaParticle c = replica_particle(a, p);
+    float sx, sy, sz, tx, ty, tz;
+    replica_normals(a.key0[r], a.key1[r], a.tag, p, n_step, c.sig, sx, sy, sz);
+    replica_stage1_math(a, c, sx, sy, sz, h1, a.h[ox], a.h[ox + 32], a.h[ox + 64], a.m[ox],
+                        a.m[ox + 32], a.m[ox + 64], tx, ty, tz);
+    a.mt[ox] = tx;
+    a.mt[ox + 32] = ty;
+    a.mt[ox + 64] = tz;
 }
 
 // stage 2 and the samples s_first .. s_first + n_rec - 1 (all recorded after
@@ -512,9 +589,13 @@ static __global__ __launch_bounds__(256) void replica_stage2_kernel(ReplicaArgs 
     const int q = live ? i : a.n * a.reps - 1;
     const int r = q / a.n, p = q % a.n;
     const size_t ox = (size_t)r * a.ld + replica_col(p, 0);
-    float mx, my, mz;
-    replica_stage2_elem(a, r, p, ox, n_step, h2, a.h[ox], a.h[ox + 32], a.h[ox + 64], live, mx,
-                        my, mz);
+    const ReplicaParticle c = replica_particle(a, p);
+    float sx, sy, sz, mx, my, mz;
+    replica_normals(a.key0[r], a.key1[r], a.tag, p, n_step, c.sig, sx, sy, sz);
+    replica_stage2_math(a, c, sx, sy, sz, h2, a.h[ox], a.h[ox + 32], a.h[ox + 64], a.mt[ox],
+                        a.mt[ox + 32], a.mt[ox + 64], a.m[ox], a.m[ox + 32], a.m[ox + 64], mx, my,
+                        mz);
+    if (live) replica_store_state(a, r, p, ox, mx, my, mz);
     for (int k = 0; k < n_rec; ++k) {
         const int64_t s = s_first + k;
         if (a.samples != nullptr && live) {

@@ -139,6 +139,28 @@ def test_512_particle_replicas_track_fp64_pair_sums():
             assert err < 1e-5, (mode, c, err)
 
 
+def test_fused_stages_equal_separate_stage_kernels(monkeypatch):
+    # The BF16x3 product runs the Heun stages in its epilogue; with
+    # MAGPY_AMD_REPLICA_FUSED=0 it writes the field and the elementwise stage
+    # kernels follow.  The field's two k halves are added in either order
+    # (commutative) and the stages evaluate the same expressions, so the
+    # states are equal bit for bit; the ensemble sums only differ in the
+    # order of their per-wave partials.
+    print("CASE replica_fused_vs_separate", flush=True)
+    e = replica_ensemble(512, 128, 5)
+    dt, S = 1e-14, 7
+    end = 150 * dt
+    res = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("MAGPY_AMD_REPLICA_FUSED", fused)
+        ses = run(e, dt, end, S, precision="fp32", dipolar="gemm_bf16x3")
+        res[fused] = ses.finish()
+    for c in ("x", "y", "z", "final_state"):
+        np.testing.assert_array_equal(res["1"][c], res["0"][c])
+    np.testing.assert_allclose(res["1"]["ensemble_m"], res["0"]["ensemble_m"], rtol=1e-12)
+    assert np.all(res["1"]["status"] == 0)
+
+
 def test_auto_selects_the_product_for_replicas():
     print("CASE replica_auto", flush=True)
     e = replica_ensemble(32, 64, 3)
