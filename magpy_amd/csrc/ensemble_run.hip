@@ -495,43 +495,53 @@ struct GroupRun : GroupBase {
     {
         if constexpr (L == 2 && !kArray && sizeof(T) == 4) {
             if (cluster_cache() == 16) {
-                launch_cluster_variant<L, kArray, 16>(s, htab);
+                if (cluster_lds_cache()) launch_cluster_variant<L, kArray, 16, 16>(s, htab);
+                else launch_cluster_variant<L, kArray, 16, 0>(s, htab);
                 return;
             }
         }
-        launch_cluster_variant<L, kArray, 0>(s, htab);
+        launch_cluster_variant<L, kArray, 0, 0>(s, htab);
     }
 
-    template <int L, bool kArray, int KC>
+    // the 16 partners after the register-cached ones keep their weights in
+    // LDS (MAGPY_AMD_CLUSTER_LDS_CACHE=0: recomputed, the same bits)
+    static bool cluster_lds_cache()
+    {
+        const char *env = std::getenv("MAGPY_AMD_CLUSTER_LDS_CACHE");
+        return !(env != nullptr && std::atoi(env) == 0);
+    }
+
+    template <int L, bool kArray, int KC, int KL>
     void launch_cluster_variant(hipStream_t s, const float *htab)
     {
         mp::ClusterArgs<T> ca{a, d_px.p, d_py.p, d_pz.p, d_w.p, d_cbase.p, d_csize.p};
-        const size_t shmem = 3 * (size_t)(block / L) * sizeof(mp::Vec4<T>);
+        const size_t shmem = 3 * (size_t)(block / L) * sizeof(mp::Vec4<T>) +
+                             (size_t)KL * (size_t)block * sizeof(float2);
         auto go = [&](auto kernel) {
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), shmem, s, ca, htab);
         };
         const bool rn = a.renorm != 0;
         if (htab) {
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true, L, kArray, KC>);
-            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false, L, kArray, KC>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, true, true, L, kArray, KC, KL>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, true, false, L, kArray, KC, KL>);
             return;
         }
         switch (shape_kind) {
         case mp::kSine:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true, L, kArray, KC>);
-            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false, L, kArray, KC>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSine, false, true, L, kArray, KC, KL>);
+            else go(mp::heun_cluster_kernel<T, mp::kSine, false, false, L, kArray, KC, KL>);
             break;
         case mp::kSquare:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true, L, kArray, KC>);
-            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false, L, kArray, KC>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kSquare, false, true, L, kArray, KC, KL>);
+            else go(mp::heun_cluster_kernel<T, mp::kSquare, false, false, L, kArray, KC, KL>);
             break;
         case mp::kConstant:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true, L, kArray, KC>);
-            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false, L, kArray, KC>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kConstant, false, true, L, kArray, KC, KL>);
+            else go(mp::heun_cluster_kernel<T, mp::kConstant, false, false, L, kArray, KC, KL>);
             break;
         default:
-            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true, L, kArray, KC>);
-            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false, L, kArray, KC>);
+            if (rn) go(mp::heun_cluster_kernel<T, mp::kMixed, false, true, L, kArray, KC, KL>);
+            else go(mp::heun_cluster_kernel<T, mp::kMixed, false, false, L, kArray, KC, KL>);
             break;
         }
     }

@@ -756,12 +756,36 @@ __device__ inline void pair_cache_init(const Vec4<T> *pos, int self, int jb, int
     }
 }
 
-// Two lanes per particle, the first K partners' geometry cached: the same
-// terms, in the same order, as dipolar_field<T, 2>.
-template <typename T, int K>
+// The next KL partners' weight w_j r^-3 and r^-2 (j = b + half + 2 (K + k)),
+// kept in LDS: geo[k * stride] is this lane's entry k (stride = lanes per
+// workgroup), computed by the same expressions as pair_cache_init.
+template <typename T, int K, int KL>
+__device__ inline void pair_geo_init(const Vec4<T> *pos, int self, int jb, int e, float2 *geo,
+                                     int stride)
+{
+#pragma clang fp contract(off)
+    const Vec4<T> pi = pos[self];
+    for (int k = 0; k < KL; ++k) {
+        const int j = jb + 2 * (K + k);
+        const bool in = j < e;
+        const Vec4<T> pj = pos[in ? j : self];
+        const bool skip = !in || j == self;
+        const T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const T r2 = skip ? T(1) : fmad(dx, dx, fmad(dy, dy, dz * dz));
+        const T rinv = __builtin_amdgcn_rsqf(r2);
+        const T rinv2 = rinv * rinv;
+        geo[k * stride] = float2{skip ? T(0) : pj.w * rinv2 * rinv, rinv2};
+    }
+}
+
+// Two lanes per particle, the first K partners' geometry cached in
+// registers and the next KL partners' weights in LDS (their d = p_j - p_i is
+// formed again from the positions): the same terms, in the same order, as
+// dipolar_field<T, 2>.
+template <typename T, int K, int KL>
 __device__ inline void dipolar_field_cached(const Vec4<T> *pos, const Vec4<T> *m, int self, int b,
-                                            int e, int half, const PairCache<T, K> &pc, T &hx,
-                                            T &hy, T &hz)
+                                            int e, int half, const PairCache<T, K> &pc,
+                                            const float2 *geo, int stride, T &hx, T &hy, T &hz)
 {
 #pragma clang fp contract(off)
     static_assert(K % 4 == 0, "cached partners come in chunks of four");
@@ -788,7 +812,31 @@ __device__ inline void dipolar_field_cached(const Vec4<T> *pos, const Vec4<T> *m
         }
     }
     const Vec4<T> pi = pos[self];
-    int j = jb + 2 * K;
+    static_assert(KL % 2 == 0, "LDS partners come in pairs");
+#pragma unroll
+    for (int c2 = 0; c2 < KL / 2; ++c2) {
+        if (jb + 2 * (K + 2 * c2) < e) {
+            Vec4<T> pj[2], mj[2];
+            float2 g[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int j = jb + 2 * (K + 2 * c2 + u);
+                const int js = j < e ? j : self;
+                pj[u] = pos[js];
+                mj[u] = m[js];
+                g[u] = geo[(2 * c2 + u) * stride];
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const T dx = pj[u].x - pi.x, dy = pj[u].y - pi.y, dz = pj[u].z - pi.z;
+                const T t = T(3) * fmad(mj[u].x, dx, fmad(mj[u].y, dy, mj[u].z * dz)) * g[u].y;
+                x = fmad(g[u].x, fmad(t, dx, -mj[u].x), x);
+                y = fmad(g[u].x, fmad(t, dy, -mj[u].y), y);
+                z = fmad(g[u].x, fmad(t, dz, -mj[u].z), z);
+            }
+        }
+    }
+    int j = jb + 2 * (K + KL);
     for (; j + 6 < e; j += 8) {
         Vec4<T> pj[4], mj[4];
 #pragma unroll
@@ -811,10 +859,12 @@ __device__ inline void dipolar_field_cached(const Vec4<T> *pos, const Vec4<T> *m
 // kArrayNoise the Wiener normals are the caller's ([step][slot][3], the
 // reference's RngArray, lib/rng.cpp:67-94) instead of the Philox stream.
 // kCache (two lanes, FP32) caches the first kCache partners' geometry per
-// lane (PairCache): workgroups of at most 256 lanes then, for the registers.
+// lane (PairCache): workgroups of at most 256 lanes then, for the registers;
+// kLds more partners keep their weights in LDS after the three state arrays
+// (kLds x lanes float2).
 template <typename T, int kShape, bool kTable, bool kRenorm, int kLanes, bool kArrayNoise,
-          int kCache = 0>
-__global__ __launch_bounds__(kCache > 0 ? 256 : 1024) void heun_cluster_kernel(
+          int kCache = 0, int kLds = 0>
+__global__ __launch_bounds__(kCache > 0 ? 256 : 1024, kLds > 0 ? 2 : 1) void heun_cluster_kernel(
     ClusterArgs<T> ca, const float *__restrict__ htab)
 {
 #pragma clang fp contract(off)
@@ -847,10 +897,13 @@ __global__ __launch_bounds__(kCache > 0 ? 256 : 1024) void heun_cluster_kernel(
     const int64_t tab0 = 4 * (n0 >> 2);
     if (lead) pos[tid] = Vec4<T>{ca.px[p], ca.py[p], ca.pz[p], ca.w[p]};
     static_assert(kCache == 0 || (kLanes == 2 && sizeof(T) == 4), "cache: two lanes, FP32");
+    static_assert(kLds == 0 || kCache > 0, "LDS partners follow the register-cached ones");
     PairCache<T, (kCache > 0 ? kCache : 4)> pc;
+    float2 *geo = reinterpret_cast<float2 *>(mb1 + slots) + threadIdx.x;
     if constexpr (kCache > 0) {
         __syncthreads();
         pair_cache_init(pos, tid, cb + half, ce, pc);
+        if constexpr (kLds > 0) pair_geo_init<T, kCache, kLds>(pos, tid, cb + half, ce, geo, blockDim.x);
     }
 
     int64_t s = a.s_next;
@@ -895,7 +948,8 @@ __global__ __launch_bounds__(kCache > 0 ? 256 : 1024) void heun_cluster_kernel(
             __syncthreads();
             T hx, hy, hz;
             if constexpr (kCache > 0)
-                dipolar_field_cached(pos, mb0, tid, cb, ce, half, pc, hx, hy, hz);
+                dipolar_field_cached<T, kCache, kLds>(pos, mb0, tid, cb, ce, half, pc, geo,
+                                                      blockDim.x, hx, hy, hz);
             else
                 dipolar_field<T, kLanes>(pos, mb0, tid, cb, ce, half, hx, hy, hz);
             T d = fmad(mx, kex, fmad(my, key, mz * kez));
@@ -908,7 +962,8 @@ __global__ __launch_bounds__(kCache > 0 ? 256 : 1024) void heun_cluster_kernel(
             if (lead) mb1[tid] = Vec4<T>{tx, ty, tz, T(0)};
             __syncthreads();
             if constexpr (kCache > 0)
-                dipolar_field_cached(pos, mb1, tid, cb, ce, half, pc, hx, hy, hz);
+                dipolar_field_cached<T, kCache, kLds>(pos, mb1, tid, cb, ce, half, pc, geo,
+                                                      blockDim.x, hx, hy, hz);
             else
                 dipolar_field<T, kLanes>(pos, mb1, tid, cb, ce, half, hx, hy, hz);
             d = fmad(tx, kex, fmad(ty, key, tz * kez));
