@@ -678,7 +678,7 @@ def grad_check_scale(args):
         run = core.Adjoint(shape[part], ax[part], ax[part], inp["seeds"][part], 400e3, 300.0, f,
                            dt, n_steps, k, 1000, prec)
         r = gradient_run(run, theta, ts, f, "scale %s group %d/%d" % (prec, g, G))
-        np.savez(args.grad_check_out, particles=part,
+        np.savez(args.grad_check_out or "grad_scale_adjoint_%s_%d.npz" % (prec, g), particles=part,
                  particle_grad_sar=r["particle_grad_sar"] * scale,
                  grad_sar=r["grad_sar"] * scale, grad_final_mz=r["grad_final_mz"] * scale,
                  sar=r["sar"] * scale, final_mz=r["final_mz"] * scale, precision=prec)
@@ -793,7 +793,7 @@ def grad_check_scale(args):
         sum32 = np.abs(adj32.sum(0) - fdc.sum(0)) / np.abs(fdc.sum(0))
         line["checked_sum_rel_err_fp32"] = sum32.tolist()
         line["sum_within_1e-3_fp32"] = bool(np.all(sum32 < 1e-3))
-    np.savez(args.grad_check_out or os.path.join(ROOT, "gpurun_out", "grad_scale_fd.npz"),
+    np.savez(args.grad_check_out or "grad_scale_fd.npz",
              sel=sel, fd=fd, adj=adj, sar_i=sar_i)
     print(json.dumps(line), flush=True)
 
