@@ -56,54 +56,87 @@ __device__ inline double dom_field(const DomArgs &a, double h, double t)
     }
 }
 
-// dom::transition_matrix (lib/dom.cpp:30-56) applied to p: dp = W p with
-// W = [[-r2, r1], [r2, -r1]]
-__device__ inline void dom_derivs(const DomArgs &a, double k, double v, double temp, double ms,
-                                  double al, double happ, double t, const double *p, double *dp)
+// The particle's constants of dom::transition_matrix (lib/dom.cpp:30-56):
+// sigma = K V / kT and the field-free part tau_N sqrt(pi) / sigma^1.5 of the
+// prefactor, in the reference's order of operations.
+struct DomParticle {
+    double happ, sigma, c0;
+};
+
+__device__ inline DomParticle dom_particle(double k, double v, double temp, double ms, double al,
+                                           double happ)
 {
 #pragma clang fp contract(off)
     const double kb = 1.38064852e-23, gamma = 1.76086e11, pi = 3.14159265358979323846;
-    const double h = dom_field(a, happ, t);
     const double sigma = k * v / kb / temp;
     const double taun = v * ms * (1 + al * al) / 2.0 / gamma / al / kb / temp;
-    const double e1 = sigma * (1 - h) * (1 - h);
-    const double e2 = sigma * (1 + h) * (1 + h);
-    const double pre = taun * sqrt(pi) / pow(sigma, 1.5) / (1 - h * h);
-    const double r1 = 1.0 / pre * (1 - h) * exp(-e1);
-    const double r2 = 1.0 / pre * (1 + h) * exp(-e2);
+    return DomParticle{happ, sigma, taun * sqrt(pi) / pow(sigma, 1.5)};
+}
+
+// The rates at time t: W = [[-r2, r1], [r2, -r1]], pre = c0 / (1 - h^2)
+// (the reference's taun sqrt(pi) / pow(sigma, 1.5) / (1 - h h), left to
+// right).  They depend on t (through the field) and the particle, not on the
+// probabilities.
+__device__ inline void dom_rates(const DomArgs &a, const DomParticle &q, double t, double &r1,
+                                 double &r2)
+{
+#pragma clang fp contract(off)
+    const double h = dom_field(a, q.happ, t);
+    const double e1 = q.sigma * (1 - h) * (1 - h);
+    const double e2 = q.sigma * (1 + h) * (1 + h);
+    const double pre = q.c0 / (1 - h * h);
+    r1 = 1.0 / pre * (1 - h) * exp(-e1);
+    r2 = 1.0 / pre * (1 + h) * exp(-e2);
+}
+
+// dp = W p
+__device__ inline void dom_apply(double r1, double r2, const double *p, double *dp)
+{
+#pragma clang fp contract(off)
     dp[0] = -r2 * p[0] + r1 * p[1];
     dp[1] = r2 * p[0] + -r1 * p[1];
 }
 
 // integrator::rk45 (lib/integrators.cpp:152-251) with the Cash-Karp tableau
 // of include/integrators.hpp:128-160; returns false if no step was accepted
-// within kDomMaxRetries attempts.
-__device__ inline bool dom_rk45(const DomArgs &a, double k, double v, double temp, double ms,
-                                double al, double happ, double *next, const double *x,
-                                double *h_ptr, double *t_ptr, double tol)
+// within kDomMaxRetries attempts.  The rates of an attempt's six stage times
+// are formed before its stages (they do not depend on the stage values), so
+// their FP64 sin/exp evaluations can overlap instead of alternating with
+// the stage arithmetic; (r1t, r2t) are the rates at t, which do not change
+// between attempts and on return hold the rates at the new t (the accepted
+// attempt's fifth stage time, t + h * 1.0 = t + h).  The same values, in
+// the same expressions, as evaluating each stage's rates in turn.
+__device__ inline bool dom_rk45(const DomArgs &a, const DomParticle &q, double *next,
+                                const double *x, double *h_ptr, double *t_ptr, double tol,
+                                double &r1t, double &r2t)
 {
 #pragma clang fp contract(off)
     double k1[2], k2[2], k3[2], k4[2], k5[2], k6[2], y[2], x5[2];
     double h = *h_ptr, t = *t_ptr, err = 0.0;
+    double r1[5], r2[5];     // stages 2..6
     bool ok = false;
+    dom_apply(r1t, r2t, x, k1);
     for (int attempt = 0; attempt < kDomMaxRetries && !ok; ++attempt) {
-        dom_derivs(a, k, v, temp, ms, al, happ, t, x, k1);
+        const double ts[5] = {t + h * 0.2, t + h * 0.3, t + h * 0.6, t + h * 1.0,
+                              t + h * (7.0 / 8.0)};
+#pragma unroll
+        for (int st = 0; st < 5; ++st) dom_rates(a, q, ts[st], r1[st], r2[st]);
         for (int i = 0; i < 2; ++i) y[i] = k1[i] * h * 0.2 + x[i];
-        dom_derivs(a, k, v, temp, ms, al, happ, t + h * 0.2, y, k2);
+        dom_apply(r1[0], r2[0], y, k2);
         for (int i = 0; i < 2; ++i) y[i] = x[i] + h * (3.0 / 40.0 * k1[i] + 9.0 / 40.0 * k2[i]);
-        dom_derivs(a, k, v, temp, ms, al, happ, t + h * 0.3, y, k3);
+        dom_apply(r1[1], r2[1], y, k3);
         for (int i = 0; i < 2; ++i)
             y[i] = x[i] + h * (3.0 / 10.0 * k1[i] + -9.0 / 10.0 * k2[i] + 6.0 / 5.0 * k3[i]);
-        dom_derivs(a, k, v, temp, ms, al, happ, t + h * 0.6, y, k4);
+        dom_apply(r1[2], r2[2], y, k4);
         for (int i = 0; i < 2; ++i)
             y[i] = x[i] + h * (-11.0 / 54.0 * k1[i] + 2.5 * k2[i] + -70.0 / 27.0 * k3[i] +
                                35.0 / 27.0 * k4[i]);
-        dom_derivs(a, k, v, temp, ms, al, happ, t + h * 1.0, y, k5);
+        dom_apply(r1[3], r2[3], y, k5);
         for (int i = 0; i < 2; ++i)
             y[i] = x[i] + h * (1631.0 / 55296.0 * k1[i] + 175.0 / 512.0 * k2[i] +
                                575.0 / 13824.0 * k3[i] + 44275.0 / 110592.0 * k4[i] +
                                253.0 / 4096.0 * k5[i]);
-        dom_derivs(a, k, v, temp, ms, al, happ, t + h * (7.0 / 8.0), y, k6);
+        dom_apply(r1[4], r2[4], y, k6);
         for (int i = 0; i < 2; ++i)
             x5[i] = x[i] + h * (37.0 / 378.0 * k1[i] + 250.0 / 621.0 * k3[i] +
                                 125.0 / 594.0 * k4[i] + 512.0 / 1771.0 * k6[i]);
@@ -130,6 +163,8 @@ __device__ inline bool dom_rk45(const DomArgs &a, double k, double v, double tem
     }
     if (!ok) return false;
     *t_ptr = t + h;
+    r1t = r1[3];
+    r2t = r2[3];
     double f = err == 0.0 ? 5.0 : 0.84 * pow(err, -0.2);
     f = f > 5 ? 5.0 : f;
     *h_ptr = f * h;
@@ -150,6 +185,9 @@ __global__ __launch_bounds__(64) void dom_kernel(DomArgs a)
     const double ts = a.end_time / (double)(a.S - 1);
     const double max_dt = a.end_time / 1000.0;
     double t = 0.0, t_last = 0.0, dt = 0.01 * a.time_step;
+    const DomParticle q = dom_particle(k, v, temp, ms, al, happ);
+    double r1t, r2t;     // rates at t
+    dom_rates(a, q, t, r1t, r2t);
     int64_t steps = 0;
     bool alive = true;
     for (int64_t s = 1; s < a.S; ++s) {
@@ -158,7 +196,7 @@ __global__ __launch_bounds__(64) void dom_kernel(DomArgs a)
             last[1] = next[1];
             t_last = t;
             if (++steps > a.max_steps ||
-                !dom_rk45(a, k, v, temp, ms, al, happ, next, last, &dt, &t, a.time_step)) {
+                !dom_rk45(a, q, next, last, &dt, &t, a.time_step, r1t, r2t)) {
                 alive = false;
                 break;
             }
