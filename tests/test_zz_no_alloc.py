@@ -58,9 +58,11 @@ def lib():
 
 
 def ensemble(sizes, seed, precision, scheme=0, interactions=False, samples=11, steps=600,
-             renorm=False, shape=0):
+             renorm=False, shape=0, replicas=False):
     """Arrays of one run: clusters of the given sizes (radii lognormal about
-    10 nm, laid along x 25 nm apart with 3 nm of jitter) in a sine field."""
+    10 nm, laid along x 25 nm apart with 3 nm of jitter) in a sine field;
+    with `replicas`, every member is the first member's cluster (the same
+    geometry, radii and axes; own seeds)."""
     rng = np.random.default_rng(seed)
     M = len(sizes)
     P = int(sum(sizes))
@@ -73,7 +75,12 @@ def ensemble(sizes, seed, precision, scheme=0, interactions=False, samples=11, s
         n = sizes[m]
         loc[offs[m]:offs[m + 1], 0] = 25e-9 * np.arange(n)
         loc[offs[m]:offs[m + 1], 1] = 3e-9 * rng.normal(size=n)
-    a = dict(offs=offs, radius=10e-9 * np.exp(0.1 * rng.normal(size=P)),
+    radius = 10e-9 * np.exp(0.1 * rng.normal(size=P))
+    if replicas:
+        n = sizes[0]
+        axes, loc, radius = (np.tile(v[:n], (M, 1)) for v in (axes, loc, radius[:, None]))
+        radius = radius[:, 0].copy()
+    a = dict(offs=offs, radius=radius,
              K=np.full(P, 23e3), axes=axes, m0=axes.copy(), loc=loc,
              ms=np.full(M, 4e5), alpha=np.full(M, 0.1), T=np.full(M, 300.0),
              shape=np.full(M, shape, dtype=np.int32), H=np.full(M, 0.02 / MU0),
@@ -126,6 +133,10 @@ CASES = [
     ("fp32 clusters 2-12", list(np.random.default_rng(4).integers(2, 13, size=25)), 0, 0, True,
      5, 500, False),
 ]
+# 64 replicas of one 40-particle cluster: the BF16x3 product path, whose
+# buffers (state, predictor, increments, tensor halves, samples) then come
+# from the fixed scratch too
+REPLICAS = ("fp32 replicas 64 x 40", [40] * 64, 0, 0, True, 5, 300, False)
 
 
 def test_noalloc_equals_allocating_entry_point():
@@ -142,6 +153,35 @@ def test_noalloc_equals_allocating_entry_point():
         assert np.all(b["status"] == 0) and b["n_steps"] > 0
         if scheme == 1:
             assert b["newton_iters"] > 0 and b["newton_failures"] == 0
+
+
+def test_noalloc_replica_product_path():
+    L = lib()
+    label, sizes, prec, scheme, inter, S, steps, renorm = REPLICAS
+    print("CASE noalloc", label, flush=True)
+    sp, keep, M, P = ensemble(sizes, 31, prec, scheme, inter, S, steps, renorm, replicas=True)
+    # the run takes the product path (MP_DIPOLAR_GEMM_BF16X3 = 2)
+    ses = C.c_void_p()
+    L.mp_session_create.argtypes = [C.POINTER(Spec), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.mp_session_info.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+    L.mp_session_destroy.argtypes = [C.c_void_p]
+    assert L.mp_session_create(C.byref(sp), 0, 0, C.byref(ses)) == 0, L.mp_last_error()
+    dipolar = C.c_int32(-1)
+    assert L.mp_session_info(ses, None, None, None, None, C.byref(dipolar), None) == 0
+    L.mp_session_destroy(ses)
+    assert dipolar.value == 2, dipolar.value
+    rc_a, a = run(L.mp_ensemble_run, sp, M, P, S)
+    assert rc_a == 0, L.mp_last_error()
+    rc_b, b = run(L.mp_ensemble_run_noalloc, sp, M, P, S)
+    assert rc_b == 0, L.mp_last_error()
+    same(a, b)
+    assert np.all(b["status"] == 0) and b["n_steps"] > 0
+    # a smaller run of the same cluster after it
+    sp, keep, M, P = ensemble([40] * 64, 32, prec, scheme, inter, 3, 120, renorm, replicas=True)
+    rc_a, a = run(L.mp_ensemble_run, sp, M, P, 3)
+    rc_b, b = run(L.mp_ensemble_run_noalloc, sp, M, P, 3)
+    assert rc_a == 0 and rc_b == 0, L.mp_last_error()
+    same(a, b)
 
 
 def test_noalloc_refuses_runs_beyond_the_scratch():
