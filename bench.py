@@ -22,6 +22,9 @@ Other workloads (--workload):
             between sample steps) and the reverse scan; gradient of a
             weighted sum of <M_z> samples with respect to (K, alpha, H,
             median r).
+  equilibrium  1,000 independent 12 nm particles along +z in zero field, Heun
+            FP64, 1e5 steps of 1e-14 s (the plain CPU-runnable ensemble; its
+            CPU baseline times whole 1e5-step runs of the oracle).
   dom       the discrete orientation model of the 65,536 AC-field particles
             (two-state master equation, adaptive Cash-Karp RK45 in FP64,
             tolerance 1e-12, 5 field cycles, 11 samples); one bench step is
@@ -85,7 +88,8 @@ SAR_ACCURACY_RECORDED = {
     "noise_tags": [0, 1], "fp32_vs_fp64_rel": [2.38e-3, 5.86e-4], "within_0.5pct": True,
     "between_thermal_histories_rel_fp64": 1.89e-2}
 DEFAULTS = {"ac_field": (65536, 131072), "clusters": (4096, 8192), "implicit": (65536, 256),
-            "replicas": (4096, 64), "adjoint": (16384, 1048000), "dom": (65536, 1)}
+            "replicas": (4096, 64), "adjoint": (16384, 1048000), "dom": (65536, 1),
+            "equilibrium": (1000, 4096)}
 
 
 def ac_field_inputs(n, seed=2024):
@@ -128,6 +132,18 @@ def replica_inputs(n_replicas, n=512, seed=2026):
                 field_frequency=3e5, seeds=rng.integers(0, 2 ** 31 - 1, size=n_replicas))
 
 
+def equilibrium_inputs(n, seed=2023):
+    """Independent 12 nm particles, easy axis and initial state along +z, no
+    applied field: the plain FP64 thermal-equilibrium ensemble."""
+    rng = np.random.default_rng(seed)
+    z = np.tile([0.0, 0.0, 1.0], (n, 1))
+    return dict(member_offsets=np.arange(n + 1, dtype=np.int64), radius=np.full(n, 12e-9),
+                anisotropy=np.full(n, 23e3), anisotropy_axis=z, magnetisation_direction=z.copy(),
+                location=np.zeros((n, 3)), magnetisation=400e3, damping=0.1, temperature=300.0,
+                field_shape='constant', field_amplitude=0.0, field_frequency=0.0,
+                seeds=rng.integers(0, 2 ** 31 - 1, size=n))
+
+
 def workload(name, n, timed_steps):
     """(inputs, session kwargs, algorithmic FLOP per time step, precision, description)."""
     if name == "ac_field":
@@ -136,6 +152,13 @@ def workload(name, n, timed_steps):
                   interactions=False, use_implicit=False)
         return (inp, kw, float(FLOP_PER_PARTICLE_STEP * n), "fp32",
                 "65,536-particle AC-field Heun ensemble (20 mT, 300 kHz, dt=1e-14 s)")
+    if name == "equilibrium":
+        inp = equilibrium_inputs(n)
+        kw = dict(time_step=1e-14, end_time=1e5 * 1e-14, max_samples=101, interactions=False,
+                  use_implicit=False)
+        return (inp, kw, float(FLOP_PER_PARTICLE_STEP * n), "fp64",
+                "1,000 independent 12 nm particles, zero field, Heun FP64 (dt=1e-14 s, "
+                "1e5 steps: the plain thermal-equilibrium ensemble)")
     if name == "implicit":
         inp = ac_field_inputs(n)
         kw = dict(time_step=5e-13, end_time=timed_steps * 5e-13, max_samples=2,
@@ -210,6 +233,8 @@ def cpu_baseline(name, inp, seconds=12.0):
     n_members = len(offs) - 1
     if name == "ac_field":
         dt, steps, implicit, interactions = 1e-14, 10000, False, False
+    elif name == "equilibrium":
+        dt, steps, implicit, interactions = 1e-14, 100000, False, False   # whole runs
     elif name == "replicas":
         dt, steps, implicit, interactions = 1e-14, 20, False, True
     elif name == "implicit":
@@ -255,7 +280,7 @@ def cpu_baseline(name, inp, seconds=12.0):
     one, m1 = run(1, seconds / 4)
     allc, ma = run(n_threads, seconds)
     what = {"ac_field": "Heun", "implicit": "implicit-midpoint", "clusters": "Heun, dipolar",
-            "replicas": "Heun, dipolar (512 particles)"}[name]
+            "replicas": "Heun, dipolar (512 particles)", "equilibrium": "Heun"}[name]
     return {"value": allc, "unit": "particle-steps/s", "cores": n_threads, "kind": "port",
             "value_1core": one, "host": host,
             "sample": "%d members x %d %s steps each (FP64 oracle in the reference's structure: "

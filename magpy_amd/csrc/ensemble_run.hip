@@ -76,7 +76,8 @@ std::vector<T> cast_vec(const std::vector<double> &v)
 // at one per CU or fewer, else 0 (32,768-step launches: 65,536 particles
 // 3.63e11 / 3.02e11 / 2.72e11 for 2 / 1 / 0; 16,384: 0.94e11 / 1.21e11 /
 // 0.68e11; 131,072: 2.99e11 / 2.82e11 / 3.66e11).  FP64: 0, the one-wave
-// kernel is faster (65,536 particles: 5.7e10 against 4.3e10).
+// kernel is faster at every size measured (65,536 particles: 5.7e10 against
+// 4.3e10; 16,384: 1.50e10 against 1.44e10; 1,000: 1.37e9 against 1.20e9).
 // MAGPY_AMD_HEUN_TWO_WAVE=0/1/2 forces the choice (2 means 1 in FP64).
 int two_wave_mode(int n_waves, bool fp32)
 {
