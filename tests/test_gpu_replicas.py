@@ -98,17 +98,18 @@ def run(e, dt, end, S, **kw):
                         trajectories=True, **e, **kw)
 
 
-def test_replica_path_matches_pair_sums():
-    print("CASE replica_vs_pairs", flush=True)
+@pytest.mark.parametrize("renorm", [False, True])
+def test_replica_path_matches_pair_sums(renorm):
+    print("CASE replica_vs_pairs renorm=%s" % renorm, flush=True)
     e = replica_ensemble(32, 64, 2)
     dt, S = 1e-14, 5
     end = 400 * dt
     res = {}
     for mode in ("gemm_f32", "gemm_bf16x3", "pairs"):
-        ses = run(e, dt, end, S, precision="fp32", dipolar=mode)
+        ses = run(e, dt, end, S, precision="fp32", dipolar=mode, renorm=renorm)
         assert ses.info()["dipolar"] == mode
         res[mode] = ses.finish()
-    ses = run(e, dt, end, S, precision="fp64")
+    ses = run(e, dt, end, S, precision="fp64", renorm=renorm)
     res["fp64"] = ses.finish()
     for mode, tol in (("gemm_f32", 2e-5), ("gemm_bf16x3", 5e-5), ("pairs", 2e-5)):
         for c in "xyz":
@@ -116,6 +117,10 @@ def test_replica_path_matches_pair_sums():
             print(mode, c, err, flush=True)
             assert err < tol, (mode, c, err)
     assert np.all(res["gemm_bf16x3"]["status"] == 0)
+    if renorm:
+        for mode in ("gemm_f32", "gemm_bf16x3"):
+            fs = res[mode]["final_state"]
+            assert np.allclose(np.linalg.norm(fs, axis=1), 1.0, atol=2e-6), mode
 
 
 def test_512_particle_replicas_track_fp64_pair_sums():
@@ -139,21 +144,22 @@ def test_512_particle_replicas_track_fp64_pair_sums():
             assert err < 1e-5, (mode, c, err)
 
 
-def test_fused_stages_equal_separate_stage_kernels(monkeypatch):
+@pytest.mark.parametrize("renorm", [False, True])
+def test_fused_stages_equal_separate_stage_kernels(monkeypatch, renorm):
     # The BF16x3 product runs the Heun stages in its epilogue; with
     # MAGPY_AMD_REPLICA_FUSED=0 it writes the field and the elementwise stage
     # kernels follow.  The field's two k halves are added in either order
     # (commutative) and the stages evaluate the same expressions, so the
     # states are equal bit for bit; the ensemble sums only differ in the
     # order of their per-wave partials.
-    print("CASE replica_fused_vs_separate", flush=True)
+    print("CASE replica_fused_vs_separate renorm=%s" % renorm, flush=True)
     e = replica_ensemble(512, 128, 5)
     dt, S = 1e-14, 7
     end = 150 * dt
     res = {}
     for fused in ("1", "0"):
         monkeypatch.setenv("MAGPY_AMD_REPLICA_FUSED", fused)
-        ses = run(e, dt, end, S, precision="fp32", dipolar="gemm_bf16x3")
+        ses = run(e, dt, end, S, precision="fp32", dipolar="gemm_bf16x3", renorm=renorm)
         res[fused] = ses.finish()
     for c in ("x", "y", "z", "final_state"):
         np.testing.assert_array_equal(res["1"][c], res["0"][c])
