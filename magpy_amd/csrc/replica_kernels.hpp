@@ -22,7 +22,8 @@
 // The product runs on MFMA in BF16 with the three-term split
 // x = x_hi + x_lo (x_hi = bf16(x), x_lo = bf16(x - x_hi)):
 //   Mt D^T ~ Mt_hi D_hi^T + Mt_hi D_lo^T + Mt_lo D_hi^T, FP32 accumulation,
-// or in FP32 on MFMA.  The Heun stages around it are elementwise kernels.
+// or in FP32 on MFMA.  The BF16x3 product runs the Heun stages in its
+// epilogue; after the FP32 product they are elementwise kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -202,9 +203,10 @@ __device__ inline void replica_store_state(const ReplicaArgs &a, int r, int p, s
 // other waits on LDS or the barrier; the halves are added through LDS at the
 // end.  Stages are double-buffered in LDS and prefetched three stages ahead
 // in registers.  Blocks are numbered so that the blocks one XCD runs (block
-// ids b with equal b % 8) share column panels of D, which then stay in that
-// XCD's L2.  rows % 128 == 0, cols % 192 == 0, K % 192 == 0 (stages come in
-// sixes).  MFMAs are ordered so that six independent accumulators separate
+// ids b with equal b % 8) share panels: with row_panels, row panels of Mt
+// (each XCD reads an eighth of Mt and all of D), else column panels of D;
+// the shared panels then stay in that XCD's L2.  rows % 128 == 0,
+// cols % 192 == 0, K % 192 == 0 (stages come in sixes).  MFMAs are ordered so that six independent accumulators separate
 // dependent ones.
 //
 // kEpi selects the epilogue: 0 stores the product Ht to c; 1 and 2 run the
